@@ -1,0 +1,41 @@
+// Shared device-side definitions of the gfx950 FMM engine.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace xfmm {
+
+enum Kind { LAPLACE = 0, HELMHOLTZ = 1, YUKAWA = 2 };
+
+constexpr double FOUR_PI_INV = 0.07957747154594766788;  // 1/(4 pi)
+constexpr int NCHILD = 8;
+constexpr int NM2L = 26;  // M2L relative positions of a target parent
+
+template <typename R>
+struct cplx {
+  R x, y;
+};
+
+// Wave numbers as seen by the kernels.  Laplace ignores both, modified
+// Helmholtz (Yukawa) uses re, Helmholtz uses re + i im.
+struct Wavek {
+  double re, im;
+};
+
+__device__ __forceinline__ double rsqrt_pos(double r2) { return r2 > 0.0 ? rsqrt(r2) : 0.0; }
+__device__ __forceinline__ float rsqrt_pos(float r2) { return r2 > 0.0f ? rsqrtf(r2) : 0.0f; }
+__device__ __forceinline__ double dexp(double x) { return exp(x); }
+__device__ __forceinline__ float dexp(float x) { return expf(x); }
+__device__ __forceinline__ void dsincos(double x, double* s, double* c) { sincos(x, s, c); }
+__device__ __forceinline__ void dsincos(float x, float* s, float* c) { sincosf(x, s, c); }
+
+// Compensated (Kahan) accumulation of a partial sum into (s, c).
+template <typename R>
+__device__ __forceinline__ void kahan_add(R& s, R& c, R v) {
+  const R y = v - c;
+  const R t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
+}  // namespace xfmm

@@ -1,0 +1,882 @@
+// C-ABI implementation of the gfx950 FMM evaluation engine
+// (include/exafmm_t_amd.h).  Owns the device copy of the flattened tree, the
+// operators and all evaluation buffers of one FMM object, and drives the
+// kernels of p2p.hip / gemm.hip / m2l.hip on one HIP stream.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "../../include/exafmm_t_amd.h"
+#include "kernels.h"
+
+using namespace xfmm;
+
+namespace {
+
+thread_local std::string g_thread_error;
+
+struct HipError {
+  std::string msg;
+};
+
+#define HIPCHK(expr)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (expr);                                                          \
+    if (e_ != hipSuccess) throw HipError{std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+struct ArgError {
+  std::string msg;
+};
+
+template <typename T>
+struct DevBuf {
+  T* ptr = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) return;
+    hipError_t e = hipMalloc(&ptr, count * sizeof(T));
+    if (e != hipSuccess) {
+      ptr = nullptr;
+      throw HipError{std::string("hipMalloc(") + std::to_string(count * sizeof(T)) + " bytes): " + hipGetErrorString(e)};
+    }
+    n = count;
+  }
+  void upload(const T* host, size_t count, hipStream_t s) {
+    alloc(count);
+    if (count) HIPCHK(hipMemcpyAsync(ptr, host, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
+};
+
+}  // namespace
+
+struct xfmm_context {
+  int kind = 0, p = 0, prec = 0, device = 0;
+  double wk_re = 0, wk_im = 0;
+  int nsurf = 0, nconv = 0, nfreq = 0, n1 = 0, vw = 1;
+  bool cplx = false;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // ---- operators
+  bool have_ops = false;
+  int nlevels = 0, nm2l = 0;
+  DevBuf<double> uc2e_uT, uc2e_vT, dc2e_uT, dc2e_vT, m2mcat, l2lcat;
+  DevBuf<double> m2l64;
+  DevBuf<float> m2l32;
+  DevBuf<double> unit_surf;
+  DevBuf<int> map_up, map_dn;
+
+  // ---- tree
+  bool have_tree = false;
+  int nnodes = 0, depth = 0, nsrcs = 0, ntrgs = 0;
+  double r0 = 0;
+  std::vector<int> h_level_offset, h_level, h_child0;
+  DevBuf<double> center, src, trg, q;
+  DevBuf<int> level, child0, src_begin, src_count, trg_begin, trg_count;
+  DevBuf<int> p2p_off, p2p_src, m2p_off, m2p_src, p2l_off, p2l_src;
+  // work lists
+  DevBuf<int> w_p2p, w_m2p, w_p2l, w_l2p_eval, w_leaf;  // w_leaf: P2M/L2P conversion leaves, sorted by level
+  int n_p2p = 0, n_m2p = 0, n_p2l = 0, n_l2p_eval = 0, n_leaf = 0;
+  std::vector<int> leaf_level_off;                        // [depth+2] ranges in w_leaf
+  DevBuf<long> leaf_slot_rows, leaf_node_rows;            // i*nsurf, node*nsurf
+  DevBuf<double> leaf_alpha;                              // 0.5^level (Laplace) or 1
+  std::vector<int> nl_level_off;                          // [depth+2] ranges of the non-leaf list
+  DevBuf<long> nl_rows, nl_child_rows;                    // node*nsurf, child0*nsurf
+  // M2L
+  int n_m2l_src = 0, n_m2l_trg = 0, n_m2l_blk = 0;
+  DevBuf<int> m2l_src_node, m2l_trg_node, m2l_blk_src, m2l_blk_trg;
+  DevBuf<double> m2l_scale;
+  std::vector<int> blk_level_off;                         // [depth+2] block ranges per target level
+  DevBuf<char> fft_in, fft_out;
+  // buffers
+  DevBuf<double> up, dn, trg_value, check, tmp;
+
+  // ---- timing
+  hipEvent_t ev[24] = {};
+  bool ev_ok = false;
+  double t_ms[11] = {0};
+
+  DevTree dev_tree() const {
+    DevTree t;
+    t.nnodes = nnodes;
+    t.depth = depth;
+    t.p = p;
+    t.nsurf = nsurf;
+    t.vw = vw;
+    t.r0 = r0;
+    t.center = center.ptr;
+    t.level = level.ptr;
+    t.child0 = child0.ptr;
+    t.src_begin = src_begin.ptr;
+    t.src_count = src_count.ptr;
+    t.trg_begin = trg_begin.ptr;
+    t.trg_count = trg_count.ptr;
+    t.src = src.ptr;
+    t.q = q.ptr;
+    t.trg = trg.ptr;
+    t.p2p_off = p2p_off.ptr;
+    t.p2p_src = p2p_src.ptr;
+    t.m2p_off = m2p_off.ptr;
+    t.m2p_src = m2p_src.ptr;
+    t.p2l_off = p2l_off.ptr;
+    t.p2l_src = p2l_src.ptr;
+    t.unit_surf = unit_surf.ptr;
+    t.up = up.ptr;
+    t.dn = dn.ptr;
+    t.trg_value = trg_value.ptr;
+    return t;
+  }
+  M2LDev m2l_dev() const {
+    M2LDev m;
+    m.p = p;
+    m.n1 = n1;
+    m.nfreq = nfreq;
+    m.nsurf = nsurf;
+    m.cplx = cplx;
+    m.mixed = prec == XFMM_MIXED;
+    m.map_up = map_up.ptr;
+    m.map_dn = map_dn.ptr;
+    m.fft_in = fft_in.ptr;
+    m.fft_out = fft_out.ptr;
+    return m;
+  }
+  Wavek wavek() const { return Wavek{wk_re, wk_im}; }
+  int op_level(int l) const { return nlevels == 1 ? 0 : l; }
+};
+
+namespace {
+
+// Unit-cube surface in the reference point order (geometry.h:21-63).
+std::vector<double> unit_surface(int p) {
+  const int n = 6 * (p - 1) * (p - 1) + 2;
+  std::vector<double> u(3 * n);
+  auto co = [&](int m) { return (2.0 * m - p + 1) / (p - 1); };
+  int c = 0;
+  auto put = [&](double a, double b, double d) { u[3 * c] = a; u[3 * c + 1] = b; u[3 * c + 2] = d; c++; };
+  put(-1, -1, -1);
+  for (int i = 0; i < p - 1; i++)
+    for (int j = 0; j < p - 1; j++) put(-1, co(i + 1), co(j));
+  for (int i = 0; i < p - 1; i++)
+    for (int j = 0; j < p - 1; j++) put(co(i), -1, co(j + 1));
+  for (int i = 0; i < p - 1; i++)
+    for (int j = 0; j < p - 1; j++) put(co(i + 1), co(j), -1);
+  for (int i = 0; i < 3 * c; i++) u[3 * c + i] = -u[i];
+  return u;
+}
+
+// surf2conv maps (geometry.h:101-134) as local indices of the p^3 corner cube
+// that holds the data: upward (p-1-x, ...) in [0,p), downward (2p-1-x, ...)
+// in [p,2p) shifted by -p.
+std::vector<int> local_map(int p, bool up) {
+  const double half = 0.5 * (p - 1);
+  std::vector<double> u = unit_surface(p);
+  const int n = int(u.size() / 3);
+  std::vector<int> m(n);
+  for (int i = 0; i < n; i++) {
+    int idx[3];
+    for (int d = 0; d < 3; d++) {
+      const double s = u[3 * i + d] * (double(p - 1) * 0.5) + half;  // lattice coordinate in [0, p-1]
+      idx[d] = up ? int((p - 1) - s) : int((2 * p - 1) - s) - p;
+    }
+    m[i] = idx[0] + p * idx[1] + p * p * idx[2];
+  }
+  return m;
+}
+
+int fail(xfmm_context* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  else g_thread_error = msg;
+  return code;
+}
+
+template <typename F>
+int guarded(xfmm_context* ctx, F&& f) {
+  if (!ctx) return fail(nullptr, XFMM_ERR_ARG, "null context");
+  try {
+    HIPCHK(hipSetDevice(ctx->device));
+    f();
+    return XFMM_OK;
+  } catch (const ArgError& e) {
+    return fail(ctx, XFMM_ERR_ARG, e.msg);
+  } catch (const HipError& e) {
+    const bool oom = e.msg.find("hipMalloc") != std::string::npos;
+    return fail(ctx, oom ? XFMM_ERR_NOMEM : XFMM_ERR_HIP, e.msg);
+  } catch (const std::bad_alloc&) {
+    return fail(ctx, XFMM_ERR_NOMEM, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(ctx, XFMM_ERR_ARG, e.what());
+  }
+}
+
+void require(bool c, const std::string& msg) {
+  if (!c) throw ArgError{msg};
+}
+
+// ---------------------------------------------------------------- operators
+
+void op_P2M(xfmm_context* c);
+void op_M2M(xfmm_context* c);
+void op_P2L(xfmm_context* c);
+void op_M2P(xfmm_context* c);
+void op_P2P(xfmm_context* c);
+void op_L2L(xfmm_context* c);
+void op_L2P(xfmm_context* c);
+
+void check_ready(xfmm_context* c) {
+  if (!c->have_tree) throw ArgError{"xfmm: tree not set (call xfmm_set_tree)"};
+  if (!c->have_ops) throw ArgError{"xfmm: operators not set (call xfmm_set_operators)"};
+  if (c->nlevels != 1 && c->nlevels < c->depth + 1)
+    throw ArgError{"xfmm: level-dependent operators cover fewer levels than the tree depth"};
+  if (c->nm2l != 1 && c->nm2l < c->depth) throw ArgError{"xfmm: M2L operators cover fewer levels than the tree depth"};
+}
+
+void op_P2M(xfmm_context* c) {
+  // check potentials of every leaf, then up = alpha * UC2E_V (UC2E_U check)
+  DevTree t = c->dev_tree();
+  launch_eval_surface(c->kind, c->prec == XFMM_MIXED, MODE_P2M, t, c->wavek(), c->w_leaf.ptr, c->n_leaf, c->check.ptr,
+                      c->stream);
+  const size_t ns2 = size_t(c->nsurf) * c->nsurf * c->vw;
+  for (int l = 0; l <= c->depth; l++) {
+    const int b = c->leaf_level_off[l], e = c->leaf_level_off[l + 1];
+    if (e <= b) continue;
+    const int L = c->op_level(l);
+    launch_gemm_rows(c->cplx, e - b, c->nsurf, c->nsurf, c->check.ptr, c->leaf_slot_rows.ptr + b,
+                     c->uc2e_uT.ptr + L * ns2, c->tmp.ptr, c->leaf_slot_rows.ptr + b, nullptr, 0, c->stream);
+    launch_gemm_rows(c->cplx, e - b, c->nsurf, c->nsurf, c->tmp.ptr, c->leaf_slot_rows.ptr + b,
+                     c->uc2e_vT.ptr + L * ns2, c->up.ptr, c->leaf_node_rows.ptr + b, c->leaf_alpha.ptr + b, 0,
+                     c->stream);
+  }
+}
+
+void op_M2M(xfmm_context* c) {
+  const size_t blk = size_t(8) * c->nsurf * c->nsurf * c->vw;
+  for (int l = c->depth - 1; l >= 0; l--) {
+    const int b = c->nl_level_off[l], e = c->nl_level_off[l + 1];
+    if (e <= b) continue;
+    const int L = c->op_level(l);
+    // parent row = [child_0 .. child_7] (contiguous, K = 8 nsurf) x M2Mcat
+    launch_gemm_rows(c->cplx, e - b, c->nsurf, 8 * c->nsurf, c->up.ptr, c->nl_child_rows.ptr + b,
+                     c->m2mcat.ptr + L * blk, c->up.ptr, c->nl_rows.ptr + b, nullptr, 1, c->stream);
+  }
+}
+
+void op_L2L(xfmm_context* c) {
+  const size_t blk = size_t(8) * c->nsurf * c->nsurf * c->vw;
+  for (int l = 0; l < c->depth; l++) {
+    const int b = c->nl_level_off[l], e = c->nl_level_off[l + 1];
+    if (e <= b) continue;
+    const int L = c->op_level(l);
+    launch_gemm_rows(c->cplx, e - b, 8 * c->nsurf, c->nsurf, c->dn.ptr, c->nl_rows.ptr + b, c->l2lcat.ptr + L * blk,
+                     c->dn.ptr, c->nl_child_rows.ptr + b, nullptr, 1, c->stream);
+  }
+}
+
+void op_L2P(xfmm_context* c) {
+  const size_t ns2 = size_t(c->nsurf) * c->nsurf * c->vw;
+  for (int l = 0; l <= c->depth; l++) {
+    const int b = c->leaf_level_off[l], e = c->leaf_level_off[l + 1];
+    if (e <= b) continue;
+    const int L = c->op_level(l);
+    launch_gemm_rows(c->cplx, e - b, c->nsurf, c->nsurf, c->dn.ptr, c->leaf_node_rows.ptr + b,
+                     c->dc2e_uT.ptr + L * ns2, c->tmp.ptr, c->leaf_slot_rows.ptr + b, nullptr, 0, c->stream);
+    launch_gemm_rows(c->cplx, e - b, c->nsurf, c->nsurf, c->tmp.ptr, c->leaf_slot_rows.ptr + b,
+                     c->dc2e_vT.ptr + L * ns2, c->dn.ptr, c->leaf_node_rows.ptr + b, c->leaf_alpha.ptr + b, 0,
+                     c->stream);
+  }
+  DevTree t = c->dev_tree();
+  launch_eval_targets(c->kind, c->prec == XFMM_MIXED, MODE_L2P, t, c->wavek(), c->w_l2p_eval.ptr, c->n_l2p_eval,
+                      c->stream);
+}
+
+void op_P2L(xfmm_context* c) {
+  DevTree t = c->dev_tree();
+  launch_eval_surface(c->kind, c->prec == XFMM_MIXED, MODE_P2L, t, c->wavek(), c->w_p2l.ptr, c->n_p2l, nullptr,
+                      c->stream);
+}
+
+void op_M2P(xfmm_context* c) {
+  DevTree t = c->dev_tree();
+  launch_eval_targets(c->kind, c->prec == XFMM_MIXED, MODE_M2P, t, c->wavek(), c->w_m2p.ptr, c->n_m2p, c->stream);
+}
+
+void op_P2P(xfmm_context* c) {
+  DevTree t = c->dev_tree();
+  launch_eval_targets(c->kind, c->prec == XFMM_MIXED, MODE_P2P, t, c->wavek(), c->w_p2p.ptr, c->n_p2p, c->stream);
+}
+
+void record(xfmm_context* c, int i) {
+  if (c->ev_ok) HIPCHK(hipEventRecord(c->ev[i], c->stream));
+}
+
+void op_M2L(xfmm_context* c, bool timed) {
+  if (c->n_m2l_trg == 0) return;
+  M2LDev m = c->m2l_dev();
+  DevTree t = c->dev_tree();
+  if (timed) record(c, 16);
+  launch_fft_up(m, t, c->m2l_src_node.ptr, c->n_m2l_src, c->stream);
+  if (timed) record(c, 17);
+  const size_t opsz = size_t(NM2L) * c->nfreq * 64 * 2;
+  for (int l = 0; l < c->depth; l++) {
+    const int b = c->blk_level_off[l], e = c->blk_level_off[l + 1];
+    if (e <= b) continue;
+    const int L = (c->nm2l == 1) ? 0 : l;
+    const void* ops = (c->prec == XFMM_MIXED) ? static_cast<const void*>(c->m2l32.ptr + L * opsz)
+                                              : static_cast<const void*>(c->m2l64.ptr + L * opsz);
+    launch_hadamard(m, ops, c->m2l_blk_src.ptr + size_t(b) * NM2L * 64, c->m2l_blk_trg.ptr + size_t(b) * 64, e - b,
+                    c->stream);
+  }
+  if (timed) record(c, 18);
+  launch_ifft_dn(m, t, c->m2l_trg_node.ptr, c->m2l_scale.ptr, c->n_m2l_trg, c->stream);
+  if (timed) record(c, 19);
+}
+
+}  // namespace
+
+// =================================================================== C ABI
+
+extern "C" {
+
+int xfmm_create(int kernel, int p, double wavek_re, double wavek_im, int precision, int device, xfmm_context** out) {
+  if (!out) return fail(nullptr, XFMM_ERR_ARG, "xfmm_create: null output pointer");
+  *out = nullptr;
+  if (kernel < XFMM_LAPLACE || kernel > XFMM_MODIFIED_HELMHOLTZ)
+    return fail(nullptr, XFMM_ERR_ARG, "xfmm_create: unknown kernel");
+  if (p < 2 || p > 16) return fail(nullptr, XFMM_ERR_ARG, "xfmm_create: expansion order p must be in [2, 16]");
+  if (precision != XFMM_FP64 && precision != XFMM_MIXED)
+    return fail(nullptr, XFMM_ERR_ARG, "xfmm_create: unknown precision mode");
+  xfmm_context* c = new (std::nothrow) xfmm_context;
+  if (!c) return fail(nullptr, XFMM_ERR_NOMEM, "xfmm_create: host allocation failed");
+  c->kind = kernel;
+  c->p = p;
+  c->prec = precision;
+  c->device = device;
+  c->wk_re = wavek_re;
+  c->wk_im = wavek_im;
+  c->cplx = kernel == XFMM_HELMHOLTZ;
+  c->vw = c->cplx ? 2 : 1;
+  c->n1 = 2 * p;
+  c->nsurf = 6 * (p - 1) * (p - 1) + 2;
+  c->nconv = c->n1 * c->n1 * c->n1;
+  c->nfreq = c->cplx ? c->nconv : c->n1 * c->n1 * (c->n1 / 2 + 1);
+  try {
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) throw ArgError{"xfmm_create: device ordinal out of range"};
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    bool ok = true;
+    for (auto& e : c->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+    c->ev_ok = ok;
+    std::vector<double> u = unit_surface(p);
+    c->unit_surf.upload(u, c->stream);
+    // the maps index the LDS cube of the transform kernels: must be a
+    // one-to-one map into [0, p^3)
+    for (int up = 0; up < 2; up++) {
+      std::vector<int> m = local_map(p, up == 1);
+      std::vector<char> seen(size_t(p) * p * p, 0);
+      for (int v : m) {
+        if (v < 0 || v >= p * p * p || seen[v]) throw ArgError{"xfmm_create: surface-to-grid map is not one-to-one"};
+        seen[v] = 1;
+      }
+      (up ? c->map_up : c->map_dn).upload(m, c->stream);
+    }
+    M2LDev m = c->m2l_dev();
+    const size_t lds = m2l_lds_bytes(m);
+    if (lds > 160 * 1024) throw ArgError{"xfmm_create: p too large for the LDS-resident M2L transforms"};
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } catch (const ArgError& e) {
+    xfmm_destroy(c);
+    return fail(nullptr, XFMM_ERR_ARG, e.msg);
+  } catch (const HipError& e) {
+    xfmm_destroy(c);
+    return fail(nullptr, XFMM_ERR_HIP, e.msg);
+  } catch (...) {
+    xfmm_destroy(c);
+    return fail(nullptr, XFMM_ERR_NOMEM, "xfmm_create: allocation failed");
+  }
+  *out = c;
+  return XFMM_OK;
+}
+
+void xfmm_destroy(xfmm_context* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;  // DevBuf destructors free device memory
+}
+
+const char* xfmm_last_error(const xfmm_context* c) { return c ? c->err.c_str() : g_thread_error.c_str(); }
+
+int xfmm_sizes(const xfmm_context* c, int* nsurf, int* nconv, int* nfreq) {
+  if (!c) return fail(nullptr, XFMM_ERR_ARG, "null context");
+  if (nsurf) *nsurf = c->nsurf;
+  if (nconv) *nconv = c->nconv;
+  if (nfreq) *nfreq = c->nfreq;
+  return XFMM_OK;
+}
+
+int xfmm_set_operators(xfmm_context* c, int nlevels, const double* uc2e_u, const double* uc2e_v,
+                       const double* dc2e_u, const double* dc2e_v, const double* m2m, const double* l2l, int nm2l,
+                       const double* m2l) {
+  return guarded(c, [&] {
+    require(nlevels >= 1 && nm2l >= 1, "xfmm_set_operators: nlevels and nm2l must be >= 1");
+    require(uc2e_u && uc2e_v && dc2e_u && dc2e_v && m2m && l2l && m2l, "xfmm_set_operators: null operator");
+    if (c->kind == XFMM_LAPLACE) require(nlevels == 1 && nm2l == 1, "xfmm_set_operators: Laplace operators are scale invariant (nlevels = nm2l = 1)");
+    const int ns = c->nsurf, vw = c->vw;
+    const size_t ns2 = size_t(ns) * ns;
+    // transposes for row-GEMM: out_row = in_row x op^T
+    auto transpose_set = [&](const double* src, std::vector<double>& dst) {
+      dst.resize(size_t(nlevels) * ns2 * vw);
+      for (int L = 0; L < nlevels; L++)
+        for (int i = 0; i < ns; i++)
+          for (int j = 0; j < ns; j++)
+            for (int w = 0; w < vw; w++)
+              dst[(L * ns2 + size_t(j) * ns + i) * vw + w] = src[(L * ns2 + size_t(i) * ns + j) * vw + w];
+    };
+    std::vector<double> h;
+    transpose_set(uc2e_u, h);
+    c->uc2e_uT.upload(h, c->stream);
+    transpose_set(uc2e_v, h);
+    c->uc2e_vT.upload(h, c->stream);
+    transpose_set(dc2e_u, h);
+    c->dc2e_uT.upload(h, c->stream);
+    transpose_set(dc2e_v, h);
+    c->dc2e_vT.upload(h, c->stream);
+    // M2Mcat[L]: (8 ns) x ns, row o*ns + k, col n = M2M[L][o][n][k]
+    // L2Lcat[L]: ns x (8 ns), row k, col o*ns + n = L2L[L][o][n][k]
+    std::vector<double> mm(size_t(nlevels) * 8 * ns2 * vw), ll(size_t(nlevels) * 8 * ns2 * vw);
+    for (int L = 0; L < nlevels; L++)
+      for (int o = 0; o < 8; o++)
+        for (int n = 0; n < ns; n++)
+          for (int k = 0; k < ns; k++)
+            for (int w = 0; w < vw; w++) {
+              const size_t s = ((size_t(L) * 8 + o) * ns2 + size_t(n) * ns + k) * vw + w;
+              mm[((size_t(L) * 8 * ns2) + (size_t(o) * ns + k) * ns + n) * vw + w] = m2m[s];
+              ll[((size_t(L) * 8 * ns2) + size_t(k) * 8 * ns + size_t(o) * ns + n) * vw + w] = l2l[s];
+            }
+    c->m2mcat.upload(mm, c->stream);
+    c->l2lcat.upload(ll, c->stream);
+    const size_t msz = size_t(nm2l) * NM2L * c->nfreq * 64 * 2;
+    if (c->prec == XFMM_MIXED) {
+      std::vector<float> f(msz);
+      for (size_t i = 0; i < msz; i++) f[i] = float(m2l[i]);
+      c->m2l32.upload(f, c->stream);
+      c->m2l64.release();
+    } else {
+      c->m2l64.upload(m2l, msz, c->stream);
+      c->m2l32.release();
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->nlevels = nlevels;
+    c->nm2l = nm2l;
+    c->have_ops = true;
+  });
+}
+
+int xfmm_set_tree(xfmm_context* c, const xfmm_tree* t) {
+  return guarded(c, [&] {
+    require(t != nullptr, "xfmm_set_tree: null tree");
+    const int nn = t->nnodes, D = t->depth;
+    require(nn >= 1 && D >= 0 && D < 30, "xfmm_set_tree: bad node count or depth");
+    require(t->level_offset && t->center && t->child0 && t->src_begin && t->src_count && t->trg_begin &&
+                t->trg_count && t->p2p_offset && t->m2p_offset && t->p2l_offset && t->m2l_source,
+            "xfmm_set_tree: null array");
+    require(t->nsrcs >= 0 && t->ntrgs >= 0, "xfmm_set_tree: negative body count");
+    require((t->nsrcs == 0 || t->src_coord) && (t->ntrgs == 0 || t->trg_coord), "xfmm_set_tree: null coordinates");
+    std::vector<int> lo(t->level_offset, t->level_offset + D + 2);
+    require(lo[0] == 0 && lo[1] == 1 && lo[D + 1] == nn, "xfmm_set_tree: level_offset must start 0,1 and end at nnodes");
+    std::vector<int> lev(nn);
+    for (int l = 0; l <= D; l++) {
+      require(lo[l + 1] >= lo[l], "xfmm_set_tree: level_offset not monotone");
+      for (int i = lo[l]; i < lo[l + 1]; i++) lev[i] = l;
+    }
+    std::vector<int> ch(t->child0, t->child0 + nn);
+    for (int i = 0; i < nn; i++) {
+      if (ch[i] < 0) continue;
+      require(ch[i] + 8 <= nn && lev[i] < D && ch[i] >= lo[lev[i] + 1] && ch[i] + 8 <= lo[lev[i] + 2],
+              "xfmm_set_tree: child block outside the next level");
+    }
+    for (int i = 0; i < nn; i++) {
+      const bool leaf = ch[i] < 0;
+      const int sb = t->src_begin[i], sc = t->src_count[i], tb = t->trg_begin[i], tc = t->trg_count[i];
+      require(sc >= 0 && tc >= 0 && sb >= 0 && tb >= 0 && sb + sc <= t->nsrcs && tb + tc <= t->ntrgs,
+              "xfmm_set_tree: body range out of bounds");
+      require(leaf || (sc == 0 && tc == 0) || true, "");
+    }
+    auto check_csr = [&](const int* off, const int* lst, const char* name, bool leaf_sources) {
+      require(off[0] == 0, std::string("xfmm_set_tree: ") + name + " offsets must start at 0");
+      for (int i = 0; i < nn; i++) require(off[i + 1] >= off[i], std::string("xfmm_set_tree: ") + name + " offsets not monotone");
+      const int m = off[nn];
+      require(m == 0 || lst, std::string("xfmm_set_tree: null ") + name + " list");
+      for (int j = 0; j < m; j++) {
+        require(lst[j] >= 0 && lst[j] < nn, std::string("xfmm_set_tree: ") + name + " entry out of range");
+        if (leaf_sources) require(ch[lst[j]] < 0, std::string("xfmm_set_tree: ") + name + " source is not a leaf");
+      }
+    };
+    check_csr(t->p2p_offset, t->p2p_source, "p2p", true);
+    check_csr(t->m2p_offset, t->m2p_source, "m2p", false);
+    check_csr(t->p2l_offset, t->p2l_source, "p2l", true);
+    for (int i = 0; i < nn; i++)
+      for (int r = 0; r < NM2L; r++) {
+        const int s = t->m2l_source[size_t(i) * NM2L + r];
+        if (s < 0) continue;
+        require(s < nn && ch[s] >= 0 && ch[i] >= 0 && lev[s] == lev[i],
+                "xfmm_set_tree: M2L pair must join two non-leaves of one level");
+      }
+
+    hipStream_t st = c->stream;
+    c->have_tree = false;
+    c->nnodes = nn;
+    c->depth = D;
+    c->nsrcs = t->nsrcs;
+    c->ntrgs = t->ntrgs;
+    c->r0 = t->r0;
+    c->h_level_offset = lo;
+    c->h_level = lev;
+    c->h_child0 = ch;
+    c->center.upload(t->center, size_t(nn) * 3, st);
+    c->level.upload(lev, st);
+    c->child0.upload(ch, st);
+    c->src_begin.upload(t->src_begin, nn, st);
+    c->src_count.upload(t->src_count, nn, st);
+    c->trg_begin.upload(t->trg_begin, nn, st);
+    c->trg_count.upload(t->trg_count, nn, st);
+    c->src.upload(t->src_coord, size_t(t->nsrcs) * 3, st);
+    c->trg.upload(t->trg_coord, size_t(t->ntrgs) * 3, st);
+    c->p2p_off.upload(t->p2p_offset, nn + 1, st);
+    c->p2p_src.upload(t->p2p_source, t->p2p_offset[nn], st);
+    c->m2p_off.upload(t->m2p_offset, nn + 1, st);
+    c->m2p_src.upload(t->m2p_source, t->m2p_offset[nn], st);
+    c->p2l_off.upload(t->p2l_offset, nn + 1, st);
+    c->p2l_src.upload(t->p2l_source, t->p2l_offset[nn], st);
+
+    // work lists
+    std::vector<int> wp2p, wm2p, wp2l, wl2p, leaf;
+    for (int i = 0; i < nn; i++) {
+      const bool isleaf = ch[i] < 0;
+      if (isleaf && t->trg_count[i] > 0) {
+        if (t->p2p_offset[i + 1] > t->p2p_offset[i]) wp2p.push_back(i);
+        if (t->m2p_offset[i + 1] > t->m2p_offset[i]) wm2p.push_back(i);
+        wl2p.push_back(i);
+      }
+      if (t->p2l_offset[i + 1] > t->p2l_offset[i]) wp2l.push_back(i);
+    }
+    // leaves for P2M/L2P conversions (reference: all leaves with bodies), by level
+    c->leaf_level_off.assign(D + 2, 0);
+    for (int l = 0; l <= D; l++) {
+      c->leaf_level_off[l] = int(leaf.size());
+      for (int i = lo[l]; i < lo[l + 1]; i++)
+        if (ch[i] < 0 && (t->src_count[i] > 0 || t->trg_count[i] > 0)) leaf.push_back(i);
+    }
+    c->leaf_level_off[D + 1] = int(leaf.size());
+    std::vector<long> lsr(leaf.size()), lnr(leaf.size());
+    std::vector<double> la(leaf.size());
+    for (size_t i = 0; i < leaf.size(); i++) {
+      lsr[i] = long(i) * c->nsurf;
+      lnr[i] = long(leaf[i]) * c->nsurf;
+      la[i] = (c->kind == XFMM_LAPLACE) ? std::ldexp(1.0, -lev[leaf[i]]) : 1.0;
+    }
+    c->w_p2p.upload(wp2p, st);
+    c->w_m2p.upload(wm2p, st);
+    c->w_p2l.upload(wp2l, st);
+    c->w_l2p_eval.upload(wl2p, st);
+    c->w_leaf.upload(leaf, st);
+    c->leaf_slot_rows.upload(lsr, st);
+    c->leaf_node_rows.upload(lnr, st);
+    c->leaf_alpha.upload(la, st);
+    c->n_p2p = int(wp2p.size());
+    c->n_m2p = int(wm2p.size());
+    c->n_p2l = int(wp2l.size());
+    c->n_l2p_eval = int(wl2p.size());
+    c->n_leaf = int(leaf.size());
+    // non-leaves by level (level-major order already)
+    std::vector<long> nr, ncr;
+    c->nl_level_off.assign(D + 2, 0);
+    for (int l = 0; l <= D; l++) {
+      c->nl_level_off[l] = int(nr.size());
+      for (int i = lo[l]; i < lo[l + 1]; i++)
+        if (ch[i] >= 0) {
+          nr.push_back(long(i) * c->nsurf);
+          ncr.push_back(long(ch[i]) * c->nsurf);
+        }
+    }
+    c->nl_level_off[D + 1] = int(nr.size());
+    c->nl_rows.upload(nr, st);
+    c->nl_child_rows.upload(ncr, st);
+
+    // M2L: source / target slots and 64-target blocks per level
+    std::vector<int> src_slot(nn, -1), trg_slot(nn, -1), srcs, trgs;
+    std::vector<double> scale;
+    for (int i = 0; i < nn; i++) {
+      bool any = false;
+      for (int r = 0; r < NM2L; r++) {
+        const int s = t->m2l_source[size_t(i) * NM2L + r];
+        if (s >= 0) {
+          any = true;
+          if (src_slot[s] < 0) {
+            src_slot[s] = 0;  // mark; numbered below in node order
+          }
+        }
+      }
+      if (any) {
+        trg_slot[i] = int(trgs.size());
+        trgs.push_back(i);
+        scale.push_back(c->kind == XFMM_LAPLACE ? std::ldexp(1.0, lev[i] + 1) : 1.0);
+      }
+    }
+    for (int i = 0; i < nn; i++)
+      if (src_slot[i] == 0 || src_slot[i] > 0) {
+        src_slot[i] = int(srcs.size());
+        srcs.push_back(i);
+      }
+    std::vector<int> bsrc, btrg;
+    c->blk_level_off.assign(D + 2, 0);
+    int nblk = 0;
+    for (int l = 0; l <= D; l++) {
+      c->blk_level_off[l] = nblk;
+      std::vector<int> lt;
+      for (int i = lo[l]; i < lo[l + 1]; i++)
+        if (trg_slot[i] >= 0) lt.push_back(i);
+      for (size_t b0 = 0; b0 < lt.size(); b0 += 64) {
+        const size_t base_s = bsrc.size(), base_t = btrg.size();
+        bsrc.resize(base_s + size_t(NM2L) * 64, -1);
+        btrg.resize(base_t + 64, -1);
+        for (size_t j = 0; j < 64 && b0 + j < lt.size(); j++) {
+          const int node = lt[b0 + j];
+          btrg[base_t + j] = trg_slot[node];
+          for (int r = 0; r < NM2L; r++) {
+            const int s = t->m2l_source[size_t(node) * NM2L + r];
+            if (s >= 0) bsrc[base_s + size_t(r) * 64 + j] = src_slot[s];
+          }
+        }
+        nblk++;
+      }
+    }
+    c->blk_level_off[D + 1] = nblk;
+    c->n_m2l_src = int(srcs.size());
+    c->n_m2l_trg = int(trgs.size());
+    c->n_m2l_blk = nblk;
+    c->m2l_src_node.upload(srcs, st);
+    c->m2l_trg_node.upload(trgs, st);
+    c->m2l_scale.upload(scale, st);
+    c->m2l_blk_src.upload(bsrc, st);
+    c->m2l_blk_trg.upload(btrg, st);
+    const size_t rb = (c->prec == XFMM_MIXED) ? 4 : 8;
+    const size_t per = size_t(c->nfreq) * NCHILD * 2 * rb;
+    c->fft_in.alloc(per * std::max(1, c->n_m2l_src));
+    c->fft_out.alloc(per * std::max(1, c->n_m2l_trg));
+
+    // buffers
+    const size_t nv = size_t(nn) * c->nsurf * c->vw;
+    c->up.alloc(nv);
+    c->dn.alloc(nv);
+    c->trg_value.alloc(std::max<size_t>(1, size_t(t->ntrgs) * 4 * c->vw));
+    c->check.alloc(std::max<size_t>(1, leaf.size() * c->nsurf * c->vw));
+    c->tmp.alloc(std::max<size_t>(1, leaf.size() * c->nsurf * c->vw));
+    c->q.alloc(std::max<size_t>(1, size_t(t->nsrcs) * c->vw));
+    launch_zero(c->q.ptr, c->q.n, st);
+    launch_zero(c->up.ptr, c->up.n, st);
+    launch_zero(c->dn.ptr, c->dn.n, st);
+    launch_zero(c->trg_value.ptr, c->trg_value.n, st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    c->have_tree = true;
+  });
+}
+
+int xfmm_set_charges_host(xfmm_context* c, const double* q) {
+  return guarded(c, [&] {
+    require(c->have_tree, "xfmm_set_charges: tree not set");
+    require(q || c->nsrcs == 0, "xfmm_set_charges: null charges");
+    if (c->nsrcs) HIPCHK(hipMemcpyAsync(c->q.ptr, q, size_t(c->nsrcs) * c->vw * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int xfmm_set_charges_device(xfmm_context* c, const void* q) {
+  return guarded(c, [&] {
+    require(c->have_tree, "xfmm_set_charges: tree not set");
+    require(q || c->nsrcs == 0, "xfmm_set_charges: null charges");
+    if (c->nsrcs) HIPCHK(hipMemcpyAsync(c->q.ptr, q, size_t(c->nsrcs) * c->vw * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  });
+}
+
+int xfmm_clear_values(xfmm_context* c) {
+  return guarded(c, [&] {
+    require(c->have_tree, "xfmm_clear_values: tree not set");
+    launch_zero(c->up.ptr, c->up.n, c->stream);
+    launch_zero(c->dn.ptr, c->dn.n, c->stream);
+    launch_zero(c->trg_value.ptr, c->trg_value.n, c->stream);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int xfmm_upward_pass(xfmm_context* c) {
+  return guarded(c, [&] {
+    check_ready(c);
+    launch_zero(c->up.ptr, c->up.n, c->stream);
+    record(c, 0);
+    op_P2M(c);
+    record(c, 1);
+    op_M2M(c);
+    record(c, 2);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int xfmm_downward_pass(xfmm_context* c) {
+  return guarded(c, [&] {
+    check_ready(c);
+    launch_zero(c->dn.ptr, c->dn.n, c->stream);
+    launch_zero(c->trg_value.ptr, c->trg_value.n, c->stream);
+    record(c, 3);
+    op_P2L(c);
+    record(c, 4);
+    op_M2P(c);
+    record(c, 5);
+    op_P2P(c);
+    record(c, 6);
+    op_M2L(c, true);
+    record(c, 7);
+    op_L2L(c);
+    record(c, 8);
+    op_L2P(c);
+    record(c, 9);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+#define XFMM_OP(NAME)                                   \
+  int xfmm_##NAME(xfmm_context* c) {                    \
+    return guarded(c, [&] {                             \
+      check_ready(c);                                   \
+      op_##NAME(c);                                     \
+      HIPCHK(hipGetLastError());                        \
+    });                                                 \
+  }
+XFMM_OP(P2M)
+XFMM_OP(M2M)
+XFMM_OP(P2L)
+XFMM_OP(M2P)
+XFMM_OP(P2P)
+XFMM_OP(L2L)
+XFMM_OP(L2P)
+int xfmm_M2L(xfmm_context* c) {
+  return guarded(c, [&] {
+    check_ready(c);
+    op_M2L(c, false);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int xfmm_get_values(xfmm_context* c, double* v) {
+  return guarded(c, [&] {
+    require(c->have_tree, "xfmm_get_values: tree not set");
+    require(v || c->ntrgs == 0, "xfmm_get_values: null output");
+    if (c->ntrgs)
+      HIPCHK(hipMemcpyAsync(v, c->trg_value.ptr, size_t(c->ntrgs) * 4 * c->vw * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int xfmm_get_equiv(xfmm_context* c, double* up, double* dn) {
+  return guarded(c, [&] {
+    require(c->have_tree, "xfmm_get_equiv: tree not set");
+    const size_t n = size_t(c->nnodes) * c->nsurf * c->vw * sizeof(double);
+    if (up) HIPCHK(hipMemcpyAsync(up, c->up.ptr, n, hipMemcpyDeviceToHost, c->stream));
+    if (dn) HIPCHK(hipMemcpyAsync(dn, c->dn.ptr, n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int xfmm_set_equiv(xfmm_context* c, const double* up, const double* dn) {
+  return guarded(c, [&] {
+    require(c->have_tree, "xfmm_set_equiv: tree not set");
+    const size_t n = size_t(c->nnodes) * c->nsurf * c->vw * sizeof(double);
+    if (up) HIPCHK(hipMemcpyAsync(c->up.ptr, up, n, hipMemcpyHostToDevice, c->stream));
+    if (dn) HIPCHK(hipMemcpyAsync(c->dn.ptr, dn, n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int xfmm_values_device(xfmm_context* c, void** ptr) {
+  return guarded(c, [&] {
+    require(c->have_tree && ptr, "xfmm_values_device: tree not set or null output");
+    *ptr = c->trg_value.ptr;
+  });
+}
+
+int xfmm_synchronize(xfmm_context* c) {
+  return guarded(c, [&] { HIPCHK(hipStreamSynchronize(c->stream)); });
+}
+
+int xfmm_timings(const xfmm_context* cc, double* ms, int cap) {
+  xfmm_context* c = const_cast<xfmm_context*>(cc);
+  if (!c || !ms) return 0;
+  if (!c->ev_ok) return 0;
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return 0;
+  // P2M M2M P2L M2P P2P M2L L2L L2P FFT HAD IFFT
+  const int pairs[11][2] = {{0, 1}, {1, 2}, {3, 4}, {4, 5}, {5, 6}, {6, 7}, {7, 8}, {8, 9}, {16, 17}, {17, 18}, {18, 19}};
+  int n = 0;
+  for (int i = 0; i < 11 && n < cap; i++) {
+    float v = 0;
+    if (hipEventElapsedTime(&v, c->ev[pairs[i][0]], c->ev[pairs[i][1]]) != hipSuccess) v = -1;
+    ms[n++] = v;
+  }
+  return n;
+}
+
+int xfmm_direct(int kernel, int precision, double wavek_re, double wavek_im, const double* src, const double* q,
+                int ns, const double* trg, int nt, double* trg_value, int gradient) {
+  if (ns < 0 || nt < 0) return fail(nullptr, XFMM_ERR_ARG, "xfmm_direct: negative size");
+  if (kernel < XFMM_LAPLACE || kernel > XFMM_MODIFIED_HELMHOLTZ) return fail(nullptr, XFMM_ERR_ARG, "xfmm_direct: unknown kernel");
+  if (nt == 0) return XFMM_OK;
+  if ((ns && (!src || !q)) || !trg || !trg_value) return fail(nullptr, XFMM_ERR_ARG, "xfmm_direct: null pointer");
+  const int vw = kernel == XFMM_HELMHOLTZ ? 2 : 1;
+  const int na = (gradient ? 4 : 1) * vw;
+  try {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipStream_t s;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    struct StreamGuard { hipStream_t s; ~StreamGuard() { (void)hipStreamDestroy(s); } } sg{s};
+    DevBuf<double> dsrc, dq, dtrg, dout, scratch;
+    dsrc.upload(src, size_t(ns) * 3, s);
+    dq.upload(q, size_t(ns) * vw, s);
+    dtrg.upload(trg, size_t(nt) * 3, s);
+    dout.upload(trg_value, size_t(nt) * na, s);
+    const int tblocks = (nt + 63) / 64;
+    int nsplit = std::max(1, std::min(2048 / tblocks + 1, (ns + 255) / 256));
+    scratch.alloc(size_t(nsplit) * nt * na);
+    launch_direct(kernel, precision == XFMM_MIXED, vw == 2, Wavek{wavek_re, wavek_im}, dsrc.ptr, dq.ptr, ns, dtrg.ptr, nt,
+                  dout.ptr, gradient, scratch.ptr, nsplit, s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(trg_value, dout.ptr, size_t(nt) * na * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  } catch (const HipError& e) {
+    return fail(nullptr, e.msg.find("hipMalloc") != std::string::npos ? XFMM_ERR_NOMEM : XFMM_ERR_HIP, e.msg);
+  } catch (...) {
+    return fail(nullptr, XFMM_ERR_NOMEM, "xfmm_direct: allocation failed");
+  }
+  return XFMM_OK;
+}
+
+}  // extern "C"

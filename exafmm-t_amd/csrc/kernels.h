@@ -1,0 +1,94 @@
+// Host-callable launchers of the gfx950 kernels (internal interface of the
+// engine library; the public interface is include/exafmm_t_amd.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "common.h"
+
+namespace xfmm {
+
+// Device view of the flattened tree and the evaluation buffers.  Values of
+// kind T are doubles (real kernels) or interleaved (re, im) doubles
+// (Helmholtz); `vw` is the number of doubles per value (1 or 2).
+struct DevTree {
+  int nnodes = 0, depth = 0, p = 0, nsurf = 0, vw = 1;
+  double r0 = 0;
+  const double* center = nullptr;  // [nnodes][3]
+  const int* level = nullptr;      // [nnodes]
+  const int* child0 = nullptr;     // [nnodes]
+  const int* src_begin = nullptr;
+  const int* src_count = nullptr;
+  const int* trg_begin = nullptr;
+  const int* trg_count = nullptr;
+  const double* src = nullptr;     // [nsrcs][3]
+  const double* q = nullptr;       // T[nsrcs]
+  const double* trg = nullptr;     // [ntrgs][3]
+  const int* p2p_off = nullptr;
+  const int* p2p_src = nullptr;
+  const int* m2p_off = nullptr;
+  const int* m2p_src = nullptr;
+  const int* p2l_off = nullptr;
+  const int* p2l_src = nullptr;
+  const double* unit_surf = nullptr;  // [nsurf][3] unit-cube surface (reference point order)
+  double* up = nullptr;               // T[nnodes][nsurf]
+  double* dn = nullptr;               // T[nnodes][nsurf]
+  double* trg_value = nullptr;        // T[ntrgs][4]
+};
+
+enum TargetMode { MODE_P2P = 0, MODE_M2P = 1, MODE_L2P = 2 };
+enum SurfaceMode { MODE_P2M = 0, MODE_P2L = 1 };
+
+// Particle targets of the leaves in work[0..nwork): P2P (sources: leaf
+// particles of the P2P list), M2P (sources: upward equivalent surfaces of the
+// M2P list, alpha 1.05), L2P (source: the leaf's own downward equivalent
+// surface, alpha 2.95).  Adds grad/potential into trg_value.
+void launch_eval_targets(int kind, bool mixed, int mode, const DevTree& t, Wavek k, const int* work, int nwork,
+                         hipStream_t s);
+
+// Surface targets: P2M writes the upward check potential of each work leaf to
+// check[slot] (slot = position in work); P2L adds the potential of the P2L list
+// sources on the downward check surface (alpha 1.05) to dn[node].
+void launch_eval_surface(int kind, bool mixed, int mode, const DevTree& t, Wavek k, const int* work, int nwork,
+                         double* check, hipStream_t s);
+
+// Direct sum of ns sources onto nt targets (potential or potential+gradient);
+// out has nt*(gradient?4:1) values of kind T and is accumulated into.
+void launch_direct(int kind, bool mixed, bool cplx, Wavek k, const double* src, const double* q, int ns,
+                   const double* trg, int nt, double* out, int gradient, double* scratch, int nsplit,
+                   hipStream_t s);
+
+// C[crow[i]] (beta=0: =, beta=1: +=) alpha[i] * A[arow[i]] x B for i < m.
+// A rows have K values, B is K x N row-major, C rows have N values (kind T).
+// arow/crow are row offsets in values (not rows).  alpha may be null (1).
+void launch_gemm_rows(bool cplx, int m, int n, int k, const double* A, const long* arow, const double* B,
+                      double* C, const long* crow, const double* alpha, int beta, hipStream_t s);
+
+// M2L stages.
+struct M2LDev {
+  int p = 0, n1 = 0, nfreq = 0, nsurf = 0;
+  bool cplx = false;                 // Helmholtz: complex c2c transforms
+  bool mixed = false;                // FP32 transforms / Hadamard
+  const int* map_up = nullptr;       // [nsurf] local index in the [0,p)^3 input cube
+  const int* map_dn = nullptr;       // [nsurf] local index in the [p,2p)^3 output cube
+  void* fft_in = nullptr;            // [nsrc_slots][nfreq][8] complex
+  void* fft_out = nullptr;           // [ntrg_slots][nfreq][8] complex
+};
+
+// fft_up_equiv: one (source slot, child) per work item; src_node[slot] gives
+// the node whose 8 children are transformed.
+void launch_fft_up(const M2LDev& m, const DevTree& t, const int* src_node, int nslots, hipStream_t s);
+// hadamard_product: targets in blocks of 64; blk_src[b][26][64] are source
+// slots (-1 = no source), blk_trg[b][64] target slots (-1 = padding).
+void launch_hadamard(const M2LDev& m, const void* m2l_ops, const int* blk_src, const int* blk_trg, int nblk,
+                     hipStream_t s);
+// ifft_dn_check: one (target slot, child) per work item, adds
+// scale[slot] * result to the child's dn vector.
+void launch_ifft_dn(const M2LDev& m, const DevTree& t, const int* trg_node, const double* scale, int nslots,
+                    hipStream_t s);
+
+// Bytes of dynamic LDS the transform kernels need for this configuration.
+size_t m2l_lds_bytes(const M2LDev& m);
+
+void launch_zero(double* p, size_t n, hipStream_t s);
+void launch_scatter_values(const double* sorted_val, const int* perm, int n, int vw, double* out, hipStream_t s);
+
+}  // namespace xfmm

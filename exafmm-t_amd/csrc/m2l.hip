@@ -1,0 +1,315 @@
+// FFT-accelerated M2L for gfx950.
+//
+// Same algorithm as exafmm-t fmm_scale_invariant.h:407-440 / fmm.h:395-446:
+//  fft_up_equiv  (fmm_scale_invariant.h:560-604): the upward equivalent
+//     densities of the 8 children of every M2L source are placed on a (2p)^3
+//     grid (surf2conv_up) and transformed, r2c for real kernels
+//     (FFTW layout, x halved: f = kx + (p+1)*(ky + 2p*kz)), c2c for Helmholtz;
+//  hadamard_product (fmm_scale_invariant.h:343-399): for every target parent,
+//     frequency k and relative position r,
+//     out[t][k][j1] += sum_j2 M[r][k][j2*8+j1] * in[s(t,r)][k][j2];
+//  ifft_dn_check (fmm_scale_invariant.h:606-646): inverse transform
+//     (unnormalised; 1/n^3 is folded into M) and gather through surf2conv_dn
+//     into the children's downward check potentials, times ifft_scale.
+//
+// GPU structure.  Transforms: one workgroup per (node, child); the input is
+// nonzero only on the [0,p)^3 corner and only the [p,2p)^3 corner of the
+// output is needed, so each 1-D pass is a small dense DFT restricted to those
+// index ranges, evaluated from LDS with a twiddle table.  Hadamard: a
+// workgroup owns 64 target parents (consecutive in Hilbert order, so their
+// sources overlap in L2) and 16 frequencies; each wave takes 4 frequencies,
+// lanes are targets, the 8x8 matrix M[r][k] is wave-uniform (scalar loads)
+// and the 26 relative positions are accumulated in registers.
+#include "kernels.h"
+
+namespace xfmm {
+
+template <typename R>
+struct C2 { R x, y; };
+
+template <typename R>
+__device__ __forceinline__ C2<R> cmul(C2<R> a, C2<R> b) {
+  return C2<R>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename R>
+__device__ __forceinline__ void cfma(C2<R>& acc, C2<R> a, C2<R> b) {
+  acc.x = fma(a.x, b.x, acc.x);
+  acc.x = fma(-a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y);
+  acc.y = fma(a.y, b.x, acc.y);
+}
+
+// twiddle table W[m] = exp(sign * 2 pi i m / n1) in LDS
+template <typename R>
+__device__ __forceinline__ void make_twiddles(C2<R>* w, int n1, int sign) {
+  for (int m = threadIdx.x; m < n1; m += blockDim.x) {
+    double s, c;
+    sincospi(double(sign) * 2.0 * m / n1, &s, &c);
+    w[m] = C2<R>{R(c), R(s)};
+  }
+}
+
+// Work item -> (slot, child) with the 8 children of a node on the same XCD:
+// consecutive workgroups are dispatched round-robin over the 8 XCDs, so the
+// children of node s use workgroup ids 64*(s/8) + 8*c + s%8.
+__device__ __forceinline__ void xcd_slot_child(int w, int nslots, int& slot, int& child) {
+  const int grp = w / 64, rem = w % 64;
+  child = rem / 8;
+  slot = grp * 8 + rem % 8;
+  (void)nslots;
+}
+
+template <typename R, bool CPLX>
+__global__ __launch_bounds__(256) void k_fft_up(M2LDev m, DevTree t, const int* __restrict__ src_node, int nslots) {
+  extern __shared__ char smem_raw[];
+  const int p = m.p, n1 = m.n1;
+  const int nh = CPLX ? n1 : n1 / 2 + 1;
+  int slot, c;
+  xcd_slot_child(blockIdx.x, nslots, slot, c);
+  if (slot >= nslots) return;
+  C2<R>* w = reinterpret_cast<C2<R>*>(smem_raw);       // n1
+  C2<R>* A = w + n1;                                   // p*p*nh
+  C2<R>* B = A + p * p * nh;                           // p*n1*nh  (shares space with the input grid)
+  R* g = reinterpret_cast<R*>(B);                      // p^3 (x2 for complex)
+  make_twiddles(w, n1, -1);
+  const int node = src_node[slot];
+  const int child = t.child0[node] + c;
+  const double* up = t.up + size_t(child) * m.nsurf * (CPLX ? 2 : 1);
+  const int p3 = p * p * p;
+  for (int e = threadIdx.x; e < p3 * (CPLX ? 2 : 1); e += blockDim.x) g[e] = R(0);
+  __syncthreads();
+  for (int i = threadIdx.x; i < m.nsurf; i += blockDim.x) {
+    const int l = m.map_up[i];
+    if (CPLX) { g[2 * l] = R(up[2 * i]); g[2 * l + 1] = R(up[2 * i + 1]); }
+    else g[l] = R(up[i]);
+  }
+  __syncthreads();
+  // x pass: A[z][y][kx] = sum_x g[z][y][x] W^(kx x)
+  for (int e = threadIdx.x; e < p * p * nh; e += blockDim.x) {
+    const int kx = e % nh, zy = e / nh;
+    C2<R> acc{0, 0};
+    for (int x = 0; x < p; x++) {
+      const C2<R> tw = w[(kx * x) % n1];
+      if (CPLX) cfma(acc, C2<R>{g[2 * (zy * p + x)], g[2 * (zy * p + x) + 1]}, tw);
+      else { const R v = g[zy * p + x]; acc.x = fma(v, tw.x, acc.x); acc.y = fma(v, tw.y, acc.y); }
+    }
+    A[e] = acc;
+  }
+  __syncthreads();
+  // y pass: B[z][ky][kx] = sum_y A[z][y][kx] W^(ky y)
+  for (int e = threadIdx.x; e < p * n1 * nh; e += blockDim.x) {
+    const int kx = e % nh, ky = (e / nh) % n1, z = e / (nh * n1);
+    C2<R> acc{0, 0};
+    for (int y = 0; y < p; y++) cfma(acc, A[(z * p + y) * nh + kx], w[(ky * y) % n1]);
+    B[e] = acc;
+  }
+  __syncthreads();
+  // z pass straight to global: f = kx + nh*(ky + n1*kz)
+  C2<R>* out = reinterpret_cast<C2<R>*>(m.fft_in) + size_t(slot) * m.nfreq * NCHILD;
+  for (int f = threadIdx.x; f < m.nfreq; f += blockDim.x) {
+    const int kx = f % nh, ky = (f / nh) % n1, kz = f / (nh * n1);
+    C2<R> acc{0, 0};
+    for (int z = 0; z < p; z++) cfma(acc, B[(z * n1 + ky) * nh + kx], w[(kz * z) % n1]);
+    out[size_t(f) * NCHILD + c] = acc;
+  }
+}
+
+template <typename R, bool CPLX>
+__global__ __launch_bounds__(256) void k_ifft_dn(M2LDev m, DevTree t, const int* __restrict__ trg_node,
+                                                 const double* __restrict__ scale, int nslots) {
+  extern __shared__ char smem_raw[];
+  const int p = m.p, n1 = m.n1;
+  const int nh = CPLX ? n1 : n1 / 2 + 1;
+  int slot, c;
+  xcd_slot_child(blockIdx.x, nslots, slot, c);
+  if (slot >= nslots) return;
+  C2<R>* w = reinterpret_cast<C2<R>*>(smem_raw);  // n1, exp(+2 pi i m/n1)
+  C2<R>* Bp = w + n1;                              // p*n1*nh : [zz][ky][kx]
+  C2<R>* Ap = Bp + p * n1 * nh;                    // p*p*nh  : [zz][yy][kx]
+  R* g = reinterpret_cast<R*>(Bp);                 // p^3 output cube (x2 complex), reuses Bp
+  make_twiddles(w, n1, +1);
+  __syncthreads();
+  const C2<R>* in = reinterpret_cast<const C2<R>*>(m.fft_out) + size_t(slot) * m.nfreq * NCHILD;
+  // inverse z pass for z' = p + zz
+  for (int e = threadIdx.x; e < p * n1 * nh; e += blockDim.x) {
+    const int kx = e % nh, ky = (e / nh) % n1, zz = e / (nh * n1);
+    const int z = p + zz;
+    C2<R> acc{0, 0};
+    for (int kz = 0; kz < n1; kz++)
+      cfma(acc, in[size_t(kx + nh * (ky + n1 * kz)) * NCHILD + c], w[(kz * z) % n1]);
+    Bp[e] = acc;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < p * p * nh; e += blockDim.x) {
+    const int kx = e % nh, yy = (e / nh) % p, zz = e / (nh * p);
+    const int y = p + yy;
+    C2<R> acc{0, 0};
+    for (int ky = 0; ky < n1; ky++) cfma(acc, Bp[(zz * n1 + ky) * nh + kx], w[(ky * y) % n1]);
+    Ap[e] = acc;
+  }
+  __syncthreads();
+  const int p3 = p * p * p;
+  for (int e = threadIdx.x; e < p3; e += blockDim.x) {
+    const int xx = e % p, zy = e / p;
+    const int x = p + xx;
+    const C2<R>* a = Ap + zy * nh;
+    if (CPLX) {
+      C2<R> acc{0, 0};
+      for (int kx = 0; kx < n1; kx++) cfma(acc, a[kx], w[(kx * x) % n1]);
+      g[2 * e] = acc.x;
+      g[2 * e + 1] = acc.y;
+    } else {
+      // c2r along x: half spectrum kx = 0..n1/2 of a Hermitian sequence
+      R acc = a[0].x + ((x & 1) ? -a[nh - 1].x : a[nh - 1].x);
+      R twice = 0;
+      for (int kx = 1; kx < nh - 1; kx++) {
+        const C2<R> tw = w[(kx * x) % n1];
+        twice = fma(a[kx].x, tw.x, twice);
+        twice = fma(-a[kx].y, tw.y, twice);
+      }
+      g[e] = acc + 2 * twice;
+    }
+  }
+  __syncthreads();
+  const int node = trg_node[slot];
+  const int child = t.child0[node] + c;
+  const double sc = scale[slot];
+  double* dn = t.dn + size_t(child) * m.nsurf * (CPLX ? 2 : 1);
+  for (int i = threadIdx.x; i < m.nsurf; i += blockDim.x) {
+    const int l = m.map_dn[i];
+    if (CPLX) {
+      dn[2 * i] += sc * double(g[2 * l]);
+      dn[2 * i + 1] += sc * double(g[2 * l + 1]);
+    } else {
+      dn[i] += sc * double(g[l]);
+    }
+  }
+}
+
+constexpr int HAD_KPW = 4;               // frequencies per wave
+constexpr int HAD_KPB = 4 * HAD_KPW;     // frequencies per workgroup
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_hadamard(M2LDev m, const R* __restrict__ ops, const int* __restrict__ blk_src,
+                                                  const int* __restrict__ blk_trg) {
+  __shared__ int src_s[NM2L * 64];
+  __shared__ unsigned long long mask_s[NM2L];
+  const int b = blockIdx.y;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (int e = threadIdx.x; e < NM2L * 64; e += 256) src_s[e] = blk_src[size_t(b) * NM2L * 64 + e];
+  __syncthreads();
+  if (threadIdx.x < NM2L) {
+    unsigned long long msk = 0;
+    for (int l = 0; l < 64; l++) if (src_s[threadIdx.x * 64 + l] >= 0) msk |= 1ull << l;
+    mask_s[threadIdx.x] = msk;
+  }
+  __syncthreads();
+  const int trg = blk_trg[size_t(b) * 64 + lane];
+  const C2<R>* in_base = reinterpret_cast<const C2<R>*>(m.fft_in);
+  C2<R>* out_base = reinterpret_cast<C2<R>*>(m.fft_out);
+  for (int j = 0; j < HAD_KPW; j++) {
+    const int k = blockIdx.x * HAD_KPB + wave * HAD_KPW + j;
+    if (k >= m.nfreq) break;
+    C2<R> acc[NCHILD];
+#pragma unroll
+    for (int i = 0; i < NCHILD; i++) acc[i] = C2<R>{0, 0};
+    for (int r = 0; r < NM2L; r++) {
+      if (mask_s[r] == 0) continue;  // no target of this block has relative position r
+      int s = src_s[r * 64 + lane];
+      const bool have = s >= 0;
+      if (!have) s = 0;
+      C2<R> v[NCHILD];
+      const C2<R>* in = in_base + (size_t(s) * m.nfreq + k) * NCHILD;
+#pragma unroll
+      for (int i = 0; i < NCHILD; i++) v[i] = have ? in[i] : C2<R>{0, 0};
+      const C2<R>* M = reinterpret_cast<const C2<R>*>(ops) + (size_t(r) * m.nfreq + k) * (NCHILD * NCHILD);
+#pragma unroll
+      for (int j2 = 0; j2 < NCHILD; j2++) {
+#pragma unroll
+        for (int j1 = 0; j1 < NCHILD; j1++) cfma(acc[j1], M[j2 * NCHILD + j1], v[j2]);
+      }
+    }
+    if (trg >= 0) {
+      C2<R>* out = out_base + (size_t(trg) * m.nfreq + k) * NCHILD;
+#pragma unroll
+      for (int i = 0; i < NCHILD; i++) out[i] = acc[i];
+    }
+  }
+}
+
+// --------------------------------------------------------------- launchers
+
+static size_t fft_up_lds(const M2LDev& m, size_t rb) {
+  const int p = m.p, n1 = m.n1, nh = m.cplx ? n1 : n1 / 2 + 1;
+  const size_t cb = 2 * rb;
+  const size_t gsz = size_t(p) * p * p * rb * (m.cplx ? 2 : 1);
+  const size_t bsz = size_t(p) * n1 * nh * cb;
+  return n1 * cb + size_t(p) * p * nh * cb + (bsz > gsz ? bsz : gsz);
+}
+static size_t ifft_dn_lds(const M2LDev& m, size_t rb) {
+  const int p = m.p, n1 = m.n1, nh = m.cplx ? n1 : n1 / 2 + 1;
+  const size_t cb = 2 * rb;
+  const size_t gsz = size_t(p) * p * p * rb * (m.cplx ? 2 : 1);
+  const size_t bsz = size_t(p) * n1 * nh * cb;
+  return n1 * cb + (bsz > gsz ? bsz : gsz) + size_t(p) * p * nh * cb;
+}
+
+size_t m2l_lds_bytes(const M2LDev& m) {
+  const size_t rb = m.mixed ? 4 : 8;
+  const size_t a = fft_up_lds(m, rb), b = ifft_dn_lds(m, rb);
+  return a > b ? a : b;
+}
+
+static int xcd_grid(int nslots) { return ((nslots + 7) / 8) * 64; }
+
+// Dynamic LDS above 64 KiB must be opted into per kernel.
+template <typename K>
+static void allow_lds(K kernel, size_t bytes) {
+  if (bytes > 64 * 1024)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              int(bytes));
+}
+
+void launch_fft_up(const M2LDev& m, const DevTree& t, const int* src_node, int nslots, hipStream_t s) {
+  if (nslots <= 0) return;
+  const size_t lds = fft_up_lds(m, m.mixed ? 4 : 8);
+  dim3 grid(xcd_grid(nslots));
+  if (m.mixed) {
+    if (m.cplx) allow_lds(k_fft_up<float, true>, lds); else allow_lds(k_fft_up<float, false>, lds);
+    if (m.cplx) hipLaunchKernelGGL((k_fft_up<float, true>), grid, dim3(256), lds, s, m, t, src_node, nslots);
+    else hipLaunchKernelGGL((k_fft_up<float, false>), grid, dim3(256), lds, s, m, t, src_node, nslots);
+  } else {
+    if (m.cplx) allow_lds(k_fft_up<double, true>, lds); else allow_lds(k_fft_up<double, false>, lds);
+    if (m.cplx) hipLaunchKernelGGL((k_fft_up<double, true>), grid, dim3(256), lds, s, m, t, src_node, nslots);
+    else hipLaunchKernelGGL((k_fft_up<double, false>), grid, dim3(256), lds, s, m, t, src_node, nslots);
+  }
+}
+
+void launch_ifft_dn(const M2LDev& m, const DevTree& t, const int* trg_node, const double* scale, int nslots,
+                    hipStream_t s) {
+  if (nslots <= 0) return;
+  const size_t lds = ifft_dn_lds(m, m.mixed ? 4 : 8);
+  dim3 grid(xcd_grid(nslots));
+  if (m.mixed) {
+    if (m.cplx) allow_lds(k_ifft_dn<float, true>, lds); else allow_lds(k_ifft_dn<float, false>, lds);
+    if (m.cplx) hipLaunchKernelGGL((k_ifft_dn<float, true>), grid, dim3(256), lds, s, m, t, trg_node, scale, nslots);
+    else hipLaunchKernelGGL((k_ifft_dn<float, false>), grid, dim3(256), lds, s, m, t, trg_node, scale, nslots);
+  } else {
+    if (m.cplx) allow_lds(k_ifft_dn<double, true>, lds); else allow_lds(k_ifft_dn<double, false>, lds);
+    if (m.cplx) hipLaunchKernelGGL((k_ifft_dn<double, true>), grid, dim3(256), lds, s, m, t, trg_node, scale, nslots);
+    else hipLaunchKernelGGL((k_ifft_dn<double, false>), grid, dim3(256), lds, s, m, t, trg_node, scale, nslots);
+  }
+}
+
+void launch_hadamard(const M2LDev& m, const void* m2l_ops, const int* blk_src, const int* blk_trg, int nblk,
+                     hipStream_t s) {
+  if (nblk <= 0) return;
+  dim3 grid((m.nfreq + HAD_KPB - 1) / HAD_KPB, nblk);
+  if (m.mixed)
+    hipLaunchKernelGGL((k_hadamard<float>), grid, dim3(256), 0, s, m, static_cast<const float*>(m2l_ops), blk_src, blk_trg);
+  else
+    hipLaunchKernelGGL((k_hadamard<double>), grid, dim3(256), 0, s, m, static_cast<const double*>(m2l_ops), blk_src, blk_trg);
+}
+
+}  // namespace xfmm

@@ -1,0 +1,382 @@
+// Particle/surface evaluation kernels for gfx950: P2P, M2P, L2P (particle
+// targets), P2M check potentials and P2L (surface targets), and the direct
+// summation used by verify().
+//
+// Kernel formulas follow exafmm-t's scalar loops (targets t, sources s,
+// d = x_s - x_t, results scaled by 1/(4 pi) at the end):
+//   Laplace   laplace.h:139-162:            phi += q/r,            grad += q d / r^3
+//   Yukawa    modified_helmholtz.h:149-170: phi += q e^{-kr}/r,    grad += q e^{-kr}(kr+1) d / r^3
+//   Helmholtz helmholtz.h:186-207:          phi += q e^{ikr}/r,    grad += (1/r^2 - ik/r) phi_ij d
+// Pairs with r = 0 contribute nothing (laplace.h:57, `invr &= r2 > zero`).
+//
+// Layout: one wave (64 lanes) per target leaf / node; lane = one target (in
+// chunks of 64); sources are staged through LDS in tiles of 64 and read as
+// wave-uniform broadcasts.  Coordinates are made relative to the target leaf
+// centre in FP64 before any FP32 conversion, so the mixed path keeps
+// |d| accuracy at the scale of the leaf.  In the FP32 path each tile is
+// summed in plain FP32 and tiles are combined with compensated (Kahan)
+// summation.
+#include "kernels.h"
+
+namespace xfmm {
+
+template <int KIND>
+constexpr bool kcplx() { return KIND == HELMHOLTZ; }
+
+// potential + gradient of one pair, accumulated into a[] (4 or 8 values)
+template <int KIND, typename R>
+__device__ __forceinline__ void pair_grad(R dx, R dy, R dz, R qr, R qi, R kr, R ki, R* a) {
+  const R r2 = dx * dx + dy * dy + dz * dz;
+  const R invr = rsqrt_pos(r2);
+  if constexpr (KIND == LAPLACE) {
+    const R qinv = qr * invr;
+    a[0] += qinv;
+    const R s = qinv * invr * invr;
+    a[1] += s * dx;
+    a[2] += s * dy;
+    a[3] += s * dz;
+  } else if constexpr (KIND == YUKAWA) {
+    const R r = r2 * invr;
+    const R qinv = qr * dexp(-kr * r) * invr;
+    a[0] += qinv;
+    const R s = qinv * (R(1) + kr * r) * invr * invr;
+    a[1] += s * dx;
+    a[2] += s * dy;
+    a[3] += s * dz;
+  } else {
+    const R r = r2 * invr;
+    const R e = dexp(-ki * r) * invr;
+    R sn, cs;
+    dsincos(kr * r, &sn, &cs);
+    const R gr = e * cs, gi = e * sn;
+    const R pr = qr * gr - qi * gi;
+    const R pim = qr * gi + qi * gr;
+    a[0] += pr;
+    a[1] += pim;
+    const R cgr = (R(1) + ki * r) * invr * invr;
+    const R cgi = -kr * invr;
+    const R fr = cgr * pr - cgi * pim;
+    const R fi = cgr * pim + cgi * pr;
+    a[2] += dx * fr;
+    a[3] += dx * fi;
+    a[4] += dy * fr;
+    a[5] += dy * fi;
+    a[6] += dz * fr;
+    a[7] += dz * fi;
+  }
+}
+
+// potential of one pair into a[0] (and a[1] for Helmholtz)
+template <int KIND, typename R>
+__device__ __forceinline__ void pair_pot(R dx, R dy, R dz, R qr, R qi, R kr, R ki, R* a) {
+  const R r2 = dx * dx + dy * dy + dz * dz;
+  const R invr = rsqrt_pos(r2);
+  if constexpr (KIND == LAPLACE) {
+    a[0] += qr * invr;
+  } else if constexpr (KIND == YUKAWA) {
+    a[0] += qr * dexp(-kr * (r2 * invr)) * invr;
+  } else {
+    const R r = r2 * invr;
+    const R e = dexp(-ki * r) * invr;
+    R sn, cs;
+    dsincos(kr * r, &sn, &cs);
+    const R gr = e * cs, gi = e * sn;
+    a[0] += qr * gr - qi * gi;
+    a[1] += qr * gi + qi * gr;
+  }
+}
+
+template <typename R, int NA>
+__device__ __forceinline__ void fold_tile(R* acc, R* comp, const R* part) {
+  if constexpr (sizeof(R) == 4) {
+#pragma unroll
+    for (int a = 0; a < NA; a++) kahan_add(acc[a], comp[a], part[a]);
+  } else {
+#pragma unroll
+    for (int a = 0; a < NA; a++) acc[a] += part[a];
+  }
+}
+
+constexpr int TILE = 64;
+
+template <int KIND, typename R, int MODE>
+__global__ __launch_bounds__(64) void k_eval_targets(DevTree t, Wavek wk, const int* __restrict__ work) {
+  constexpr bool C = kcplx<KIND>();
+  constexpr int NA = C ? 8 : 4;
+  __shared__ R sx[TILE], sy[TILE], sz[TILE], sq[TILE], sqi[TILE];
+  const int leaf = work[blockIdx.x];
+  const int lane = threadIdx.x;
+  const double cx = t.center[3 * leaf], cy = t.center[3 * leaf + 1], cz = t.center[3 * leaf + 2];
+  const int tb = t.trg_begin[leaf], nt = t.trg_count[leaf];
+  const R kr = R(wk.re), ki = R(wk.im);
+  const int vw = t.vw;
+  int g0 = 0, g1 = 1;
+  if (MODE == MODE_P2P) { g0 = t.p2p_off[leaf]; g1 = t.p2p_off[leaf + 1]; }
+  if (MODE == MODE_M2P) { g0 = t.m2p_off[leaf]; g1 = t.m2p_off[leaf + 1]; }
+  for (int t0 = 0; t0 < nt; t0 += TILE) {
+    const int ti = t0 + lane;
+    const bool act = ti < nt;
+    R tx = 0, ty = 0, tz = 0;
+    if (act) {
+      const double* x = t.trg + 3 * size_t(tb + ti);
+      tx = R(x[0] - cx);
+      ty = R(x[1] - cy);
+      tz = R(x[2] - cz);
+    }
+    R acc[NA], comp[NA];
+#pragma unroll
+    for (int a = 0; a < NA; a++) { acc[a] = 0; comp[a] = 0; }
+    for (int g = g0; g < g1; g++) {
+      int snode, ns, sb = 0;
+      double b = 0, ox = 0, oy = 0, oz = 0;
+      const double* eq = nullptr;
+      if (MODE == MODE_P2P) {
+        snode = t.p2p_src[g];
+        sb = t.src_begin[snode];
+        ns = t.src_count[snode];
+      } else {
+        snode = (MODE == MODE_M2P) ? t.m2p_src[g] : leaf;
+        ns = t.nsurf;
+        b = ((MODE == MODE_M2P) ? 1.05 : 2.95) * t.r0 * ldexp(1.0, -t.level[snode]);
+        ox = t.center[3 * snode] - cx;
+        oy = t.center[3 * snode + 1] - cy;
+        oz = t.center[3 * snode + 2] - cz;
+        eq = ((MODE == MODE_M2P) ? t.up : t.dn) + size_t(snode) * t.nsurf * vw;
+      }
+      for (int s0 = 0; s0 < ns; s0 += TILE) {
+        const int m = min(TILE, ns - s0);
+        __syncthreads();
+        if (lane < m) {
+          const int si = s0 + lane;
+          if (MODE == MODE_P2P) {
+            const double* x = t.src + 3 * size_t(sb + si);
+            sx[lane] = R(x[0] - cx);
+            sy[lane] = R(x[1] - cy);
+            sz[lane] = R(x[2] - cz);
+            const double* qq = t.q + size_t(sb + si) * vw;
+            sq[lane] = R(qq[0]);
+            if (C) sqi[lane] = R(qq[1]);
+          } else {
+            sx[lane] = R(ox + t.unit_surf[3 * si] * b);
+            sy[lane] = R(oy + t.unit_surf[3 * si + 1] * b);
+            sz[lane] = R(oz + t.unit_surf[3 * si + 2] * b);
+            sq[lane] = R(eq[size_t(si) * vw]);
+            if (C) sqi[lane] = R(eq[size_t(si) * vw + 1]);
+          }
+        }
+        __syncthreads();
+        R part[NA];
+#pragma unroll
+        for (int a = 0; a < NA; a++) part[a] = 0;
+        for (int j = 0; j < m; j++)
+          pair_grad<KIND, R>(sx[j] - tx, sy[j] - ty, sz[j] - tz, sq[j], C ? sqi[j] : R(0), kr, ki, part);
+        fold_tile<R, NA>(acc, comp, part);
+      }
+    }
+    if (act) {
+      double* o = t.trg_value + size_t(tb + ti) * 4 * vw;
+#pragma unroll
+      for (int a = 0; a < NA; a++) o[a] += (double(acc[a]) - double(comp[a])) * FOUR_PI_INV;
+    }
+  }
+}
+
+template <int KIND, typename R, int MODE>
+__global__ __launch_bounds__(64) void k_eval_surface(DevTree t, Wavek wk, const int* __restrict__ work,
+                                                     double* __restrict__ check) {
+  constexpr bool C = kcplx<KIND>();
+  constexpr int NA = C ? 2 : 1;
+  __shared__ R sx[TILE], sy[TILE], sz[TILE], sq[TILE], sqi[TILE];
+  const int node = work[blockIdx.x];
+  const int lane = threadIdx.x;
+  const double cx = t.center[3 * node], cy = t.center[3 * node + 1], cz = t.center[3 * node + 2];
+  const double b = ((MODE == MODE_P2M) ? 2.95 : 1.05) * t.r0 * ldexp(1.0, -t.level[node]);
+  const R kr = R(wk.re), ki = R(wk.im);
+  const int vw = t.vw;
+  int g0 = 0, g1 = 1;
+  if (MODE == MODE_P2L) { g0 = t.p2l_off[node]; g1 = t.p2l_off[node + 1]; }
+  for (int i0 = 0; i0 < t.nsurf; i0 += TILE) {
+    const int i = i0 + lane;
+    const bool act = i < t.nsurf;
+    R px = 0, py = 0, pz = 0;
+    if (act) {
+      px = R(t.unit_surf[3 * i] * b);
+      py = R(t.unit_surf[3 * i + 1] * b);
+      pz = R(t.unit_surf[3 * i + 2] * b);
+    }
+    R acc[NA], comp[NA];
+#pragma unroll
+    for (int a = 0; a < NA; a++) { acc[a] = 0; comp[a] = 0; }
+    for (int g = g0; g < g1; g++) {
+      const int snode = (MODE == MODE_P2L) ? t.p2l_src[g] : node;
+      const int sb = t.src_begin[snode], ns = t.src_count[snode];
+      for (int s0 = 0; s0 < ns; s0 += TILE) {
+        const int m = min(TILE, ns - s0);
+        __syncthreads();
+        if (lane < m) {
+          const double* x = t.src + 3 * size_t(sb + s0 + lane);
+          sx[lane] = R(x[0] - cx);
+          sy[lane] = R(x[1] - cy);
+          sz[lane] = R(x[2] - cz);
+          const double* qq = t.q + size_t(sb + s0 + lane) * vw;
+          sq[lane] = R(qq[0]);
+          if (C) sqi[lane] = R(qq[1]);
+        }
+        __syncthreads();
+        R part[NA];
+#pragma unroll
+        for (int a = 0; a < NA; a++) part[a] = 0;
+        for (int j = 0; j < m; j++)
+          pair_pot<KIND, R>(sx[j] - px, sy[j] - py, sz[j] - pz, sq[j], C ? sqi[j] : R(0), kr, ki, part);
+        fold_tile<R, NA>(acc, comp, part);
+      }
+    }
+    if (act) {
+      if (MODE == MODE_P2M) {
+        double* o = check + (size_t(blockIdx.x) * t.nsurf + i) * vw;
+#pragma unroll
+        for (int a = 0; a < NA; a++) o[a] = (double(acc[a]) - double(comp[a])) * FOUR_PI_INV;
+      } else {
+        double* o = t.dn + (size_t(node) * t.nsurf + i) * vw;
+#pragma unroll
+        for (int a = 0; a < NA; a++) o[a] += (double(acc[a]) - double(comp[a])) * FOUR_PI_INV;
+      }
+    }
+  }
+}
+
+// Direct sum: block (x = 64 targets, y = source split).  Differences are
+// formed in FP64; tiles are combined with compensated summation in both
+// precisions (this kernel is the repository's accuracy check).
+template <int KIND, typename R, bool GRAD>
+__global__ __launch_bounds__(64) void k_direct(const double* __restrict__ src, const double* __restrict__ q, int ns,
+                                               const double* __restrict__ trg, int nt, double* __restrict__ partial,
+                                               int chunk, Wavek wk) {
+  constexpr bool C = kcplx<KIND>();
+  constexpr int NA = GRAD ? (C ? 8 : 4) : (C ? 2 : 1);
+  constexpr int vw = C ? 2 : 1;
+  __shared__ double sx[TILE], sy[TILE], sz[TILE];
+  __shared__ R sq[TILE], sqi[TILE];
+  const int lane = threadIdx.x;
+  const int ti = blockIdx.x * TILE + lane;
+  const bool act = ti < nt;
+  double tx = 0, ty = 0, tz = 0;
+  if (act) { tx = trg[3 * size_t(ti)]; ty = trg[3 * size_t(ti) + 1]; tz = trg[3 * size_t(ti) + 2]; }
+  const int sbeg = blockIdx.y * chunk;
+  const int send = min(ns, sbeg + chunk);
+  const R kr = R(wk.re), ki = R(wk.im);
+  R acc[NA], comp[NA];
+#pragma unroll
+  for (int a = 0; a < NA; a++) { acc[a] = 0; comp[a] = 0; }
+  for (int s0 = sbeg; s0 < send; s0 += TILE) {
+    const int m = min(TILE, send - s0);
+    __syncthreads();
+    if (lane < m) {
+      const size_t si = size_t(s0 + lane);
+      sx[lane] = src[3 * si];
+      sy[lane] = src[3 * si + 1];
+      sz[lane] = src[3 * si + 2];
+      sq[lane] = R(q[si * vw]);
+      if (C) sqi[lane] = R(q[si * vw + 1]);
+    }
+    __syncthreads();
+    R part[NA];
+#pragma unroll
+    for (int a = 0; a < NA; a++) part[a] = 0;
+    for (int j = 0; j < m; j++) {
+      const R dx = R(sx[j] - tx), dy = R(sy[j] - ty), dz = R(sz[j] - tz);
+      if (GRAD) pair_grad<KIND, R>(dx, dy, dz, sq[j], C ? sqi[j] : R(0), kr, ki, part);
+      else pair_pot<KIND, R>(dx, dy, dz, sq[j], C ? sqi[j] : R(0), kr, ki, part);
+    }
+#pragma unroll
+    for (int a = 0; a < NA; a++) kahan_add(acc[a], comp[a], part[a]);
+  }
+  if (act) {
+    double* o = partial + (size_t(blockIdx.y) * nt + ti) * NA;
+#pragma unroll
+    for (int a = 0; a < NA; a++) o[a] = double(acc[a]) - double(comp[a]);
+  }
+}
+
+__global__ void k_direct_reduce(const double* __restrict__ partial, int nsplit, int nt, int na,
+                                double* __restrict__ out) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= size_t(nt) * na) return;
+  double s = 0, c = 0;
+  for (int k = 0; k < nsplit; k++) kahan_add(s, c, partial[size_t(k) * nt * na + i]);
+  out[i] += (s - c) * FOUR_PI_INV;
+}
+
+// ---------------------------------------------------------------- launchers
+
+template <int KIND, typename R>
+static void eval_targets_t(int mode, const DevTree& t, Wavek k, const int* work, int nwork, hipStream_t s) {
+  if (nwork <= 0) return;
+  if (mode == MODE_P2P) hipLaunchKernelGGL((k_eval_targets<KIND, R, MODE_P2P>), dim3(nwork), dim3(64), 0, s, t, k, work);
+  else if (mode == MODE_M2P) hipLaunchKernelGGL((k_eval_targets<KIND, R, MODE_M2P>), dim3(nwork), dim3(64), 0, s, t, k, work);
+  else hipLaunchKernelGGL((k_eval_targets<KIND, R, MODE_L2P>), dim3(nwork), dim3(64), 0, s, t, k, work);
+}
+
+void launch_eval_targets(int kind, bool mixed, int mode, const DevTree& t, Wavek k, const int* work, int nwork,
+                         hipStream_t s) {
+  // Only the near field (P2P) runs in FP32 in the mixed path; the equivalent
+  // surfaces (M2P, L2P) carry the FP64 pseudo-inverse output and stay FP64.
+  const bool f32 = mixed && mode == MODE_P2P;
+  switch (kind) {
+    case LAPLACE: f32 ? eval_targets_t<LAPLACE, float>(mode, t, k, work, nwork, s)
+                      : eval_targets_t<LAPLACE, double>(mode, t, k, work, nwork, s); break;
+    case YUKAWA: f32 ? eval_targets_t<YUKAWA, float>(mode, t, k, work, nwork, s)
+                     : eval_targets_t<YUKAWA, double>(mode, t, k, work, nwork, s); break;
+    default: f32 ? eval_targets_t<HELMHOLTZ, float>(mode, t, k, work, nwork, s)
+                 : eval_targets_t<HELMHOLTZ, double>(mode, t, k, work, nwork, s); break;
+  }
+}
+
+template <int KIND>
+static void eval_surface_t(int mode, const DevTree& t, Wavek k, const int* work, int nwork, double* check,
+                           hipStream_t s) {
+  if (nwork <= 0) return;
+  if (mode == MODE_P2M) hipLaunchKernelGGL((k_eval_surface<KIND, double, MODE_P2M>), dim3(nwork), dim3(64), 0, s, t, k, work, check);
+  else hipLaunchKernelGGL((k_eval_surface<KIND, double, MODE_P2L>), dim3(nwork), dim3(64), 0, s, t, k, work, check);
+}
+
+void launch_eval_surface(int kind, bool mixed, int mode, const DevTree& t, Wavek k, const int* work, int nwork,
+                         double* check, hipStream_t s) {
+  (void)mixed;  // check potentials feed the FP64 pseudo-inverse solves: always FP64
+  switch (kind) {
+    case LAPLACE: eval_surface_t<LAPLACE>(mode, t, k, work, nwork, check, s); break;
+    case YUKAWA: eval_surface_t<YUKAWA>(mode, t, k, work, nwork, check, s); break;
+    default: eval_surface_t<HELMHOLTZ>(mode, t, k, work, nwork, check, s); break;
+  }
+}
+
+template <int KIND, typename R>
+static void direct_t(Wavek k, const double* src, const double* q, int ns, const double* trg, int nt, double* out,
+                     int gradient, double* scratch, int nsplit, hipStream_t s) {
+  const int na = (gradient ? 4 : 1) * (KIND == HELMHOLTZ ? 2 : 1);
+  int chunk = (ns + nsplit - 1) / nsplit;
+  chunk = ((chunk + TILE - 1) / TILE) * TILE;
+  if (chunk == 0) chunk = TILE;
+  dim3 grid((nt + TILE - 1) / TILE, nsplit);
+  if (gradient) hipLaunchKernelGGL((k_direct<KIND, R, true>), grid, dim3(64), 0, s, src, q, ns, trg, nt, scratch, chunk, k);
+  else hipLaunchKernelGGL((k_direct<KIND, R, false>), grid, dim3(64), 0, s, src, q, ns, trg, nt, scratch, chunk, k);
+  const size_t n = size_t(nt) * na;
+  hipLaunchKernelGGL(k_direct_reduce, dim3((n + 255) / 256), dim3(256), 0, s, scratch, nsplit, nt, na, out);
+}
+
+void launch_direct(int kind, bool mixed, bool cplx, Wavek k, const double* src, const double* q, int ns,
+                   const double* trg, int nt, double* out, int gradient, double* scratch, int nsplit,
+                   hipStream_t s) {
+  (void)cplx;
+  if (nt <= 0) return;
+  switch (kind) {
+    case LAPLACE: mixed ? direct_t<LAPLACE, float>(k, src, q, ns, trg, nt, out, gradient, scratch, nsplit, s)
+                        : direct_t<LAPLACE, double>(k, src, q, ns, trg, nt, out, gradient, scratch, nsplit, s); break;
+    case YUKAWA: mixed ? direct_t<YUKAWA, float>(k, src, q, ns, trg, nt, out, gradient, scratch, nsplit, s)
+                       : direct_t<YUKAWA, double>(k, src, q, ns, trg, nt, out, gradient, scratch, nsplit, s); break;
+    default: mixed ? direct_t<HELMHOLTZ, float>(k, src, q, ns, trg, nt, out, gradient, scratch, nsplit, s)
+                   : direct_t<HELMHOLTZ, double>(k, src, q, ns, trg, nt, out, gradient, scratch, nsplit, s); break;
+  }
+}
+
+}  // namespace xfmm

@@ -1,0 +1,161 @@
+/*
+ * exafmm_t_amd.h -- C ABI of the MI355X (gfx950) FMM evaluation engine.
+ *
+ * The engine replaces the evaluation phase of exafmm-t, i.e.
+ * FmmBase<T>::upward_pass / downward_pass (reference include/fmm_base.h:162-200)
+ * and the operators they call.  Everything that exafmm-t does before the
+ * evaluation (tree, 2:1 balance, lists, precompute, M2L_setup) stays on the
+ * host and is handed over once through xfmm_set_operators / xfmm_set_tree.
+ *
+ * Conventions
+ *  - Plain C types only.  All host pointers are read during the call and may
+ *    be freed afterwards.  "T" below means double for the real kernels
+ *    (Laplace, modified Helmholtz) and interleaved (re, im) double pairs for
+ *    the Helmholtz kernel.
+ *  - Every function returns XFMM_OK (0) or a positive status code; nothing is
+ *    thrown across the ABI.  xfmm_last_error() returns the message of the last
+ *    failure of that context (or of the calling thread for context creation).
+ *  - One context = one HIP stream on one device.  Contexts are independent;
+ *    calls on one context must not overlap.
+ *  - Sorted order: sources/targets in the order given by xfmm_tree (grouped by
+ *    leaf); the engine never sees the caller's original body order.
+ */
+#ifndef EXAFMM_T_AMD_H
+#define EXAFMM_T_AMD_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { XFMM_LAPLACE = 0, XFMM_HELMHOLTZ = 1, XFMM_MODIFIED_HELMHOLTZ = 2 };
+enum { XFMM_FP64 = 0, XFMM_MIXED = 1 };
+enum {
+  XFMM_OK = 0,
+  XFMM_ERR_ARG = 1,    /* invalid argument / inconsistent tree */
+  XFMM_ERR_STATE = 2,  /* call order violated (e.g. evaluate before set_tree) */
+  XFMM_ERR_HIP = 3,    /* HIP runtime error (message has the HIP error string) */
+  XFMM_ERR_NOMEM = 4   /* device allocation failed */
+};
+
+typedef struct xfmm_context xfmm_context;
+
+/* Flattened octree.  Node order is level-major: nodes of level l are
+ * [level_offset[l], level_offset[l+1]).  The 8 children of a non-leaf n are
+ * the consecutive nodes child0[n] .. child0[n]+7 in octant order (x bit 0,
+ * y bit 1, z bit 2).  Leaves have child0 = -1.  Lists are CSR arrays indexed
+ * by target node: list entries of node n are list[off[n] .. off[n+1]).
+ * Replaces the Node pointer graph of the reference (include/exafmm_t.h:102-129,
+ * lists built by include/build_list.h:16-197, M2L_setup of
+ * include/fmm_scale_invariant.h:277-341 / include/fmm.h:265-333). */
+typedef struct {
+  int nnodes;
+  int depth;                 /* deepest level; FmmBase::depth */
+  double r0;                 /* half side of the root box; FmmBase::r0 */
+  double x0[3];              /* centre of the root box */
+  const int* level_offset;   /* [depth + 2] */
+  const double* center;      /* [nnodes][3] */
+  const int* child0;         /* [nnodes] */
+  const int* src_begin;      /* [nnodes] leaves: first sorted source; else 0 */
+  const int* src_count;      /* [nnodes] leaves: number of sources; else 0 */
+  const int* trg_begin;      /* [nnodes] */
+  const int* trg_count;      /* [nnodes] */
+  int nsrcs;
+  int ntrgs;
+  const double* src_coord;   /* [nsrcs][3], sorted by leaf */
+  const double* trg_coord;   /* [ntrgs][3], sorted by leaf */
+  const int* p2p_offset;     /* [nnodes + 1]; sources: leaf nodes (Node::P2P_list) */
+  const int* p2p_source;
+  const int* m2p_offset;     /* [nnodes + 1]; sources: any node (Node::M2P_list) */
+  const int* m2p_source;
+  const int* p2l_offset;     /* [nnodes + 1]; sources: leaf nodes (Node::P2L_list) */
+  const int* p2l_source;
+  const int* m2l_source;     /* [nnodes][26]; source node per M2L relpos or -1 (Node::M2L_list) */
+} xfmm_tree;
+
+/* Create a context (FmmBase ctor: p, kernel, wavek; reference
+ * include/laplace.h:15, include/helmholtz.h:16, include/modified_helmholtz.h:17).
+ * wavek_im is used by the Helmholtz kernel only.  device = HIP device ordinal. */
+int xfmm_create(int kernel, int p, double wavek_re, double wavek_im, int precision, int device,
+                xfmm_context** out);
+void xfmm_destroy(xfmm_context* ctx);
+const char* xfmm_last_error(const xfmm_context* ctx); /* ctx may be NULL */
+
+/* Derived sizes (reference include/fmm_base.h:27-36). */
+int xfmm_sizes(const xfmm_context* ctx, int* nsurf, int* nconv, int* nfreq);
+
+/* Precomputed operators (outputs of precompute(); reference
+ * include/fmm_scale_invariant.h:151-160, include/fmm.h:141-153).
+ * nlevels = 1 for Laplace (scale invariant), depth + 1 otherwise.
+ *   uc2e_u, uc2e_v, dc2e_u, dc2e_v : T[nlevels][nsurf][nsurf] (row-major)
+ *   m2m, l2l                       : T[nlevels][8][nsurf][nsurf]
+ *   m2l                            : double[nm2l][26][nfreq][8*8][2]
+ *     (layout of matrix_M2L, fmm_scale_invariant.h:545-552); nm2l = 1 for
+ *     Laplace, depth for the others (slot l serves target level l). */
+int xfmm_set_operators(xfmm_context* ctx, int nlevels, const double* uc2e_u, const double* uc2e_v,
+                       const double* dc2e_u, const double* dc2e_v, const double* m2m, const double* l2l,
+                       int nm2l, const double* m2l);
+
+/* Upload the tree once (validates the layout; returns XFMM_ERR_ARG on a
+ * malformed tree).  Charges are reset to zero. */
+int xfmm_set_tree(xfmm_context* ctx, const xfmm_tree* tree);
+
+/* Sorted charges, T[nsrcs].  Host pointer (..._host) or device pointer
+ * already resident in HBM (..._device, same layout). */
+int xfmm_set_charges_host(xfmm_context* ctx, const double* q);
+int xfmm_set_charges_device(xfmm_context* ctx, const void* q_device);
+
+/* Zero target values and equivalent/check vectors (python clear_values,
+ * reference python/exafmm.cpp:313-323). */
+int xfmm_clear_values(xfmm_context* ctx);
+
+/* FmmBase::upward_pass / downward_pass (include/fmm_base.h:162-200).  Each
+ * pass starts by zeroing what it produces (up_equiv, resp. dn_equiv and the
+ * target values), so repeated evaluation does not need clear_values. */
+int xfmm_upward_pass(xfmm_context* ctx);
+int xfmm_downward_pass(xfmm_context* ctx);
+
+/* Individual operators, same order and semantics as the reference
+ * (accumulating into the device buffers; no zeroing):
+ *   P2M  fmm_scale_invariant.h:163-195 / fmm.h:156-185
+ *   M2M  fmm_scale_invariant.h:233-253 / fmm.h:221-241 (all levels, bottom-up)
+ *   P2L  fmm_base.h:127-154     M2P  fmm_base.h:97-124    P2P  fmm_base.h:82-94
+ *   M2L  fmm_scale_invariant.h:407-440 / fmm.h:395-446
+ *   L2L  fmm_scale_invariant.h:256-275 / fmm.h:244-263 (all levels, top-down)
+ *   L2P  fmm_scale_invariant.h:198-230 / fmm.h:188-218 */
+int xfmm_P2M(xfmm_context* ctx);
+int xfmm_M2M(xfmm_context* ctx);
+int xfmm_P2L(xfmm_context* ctx);
+int xfmm_M2P(xfmm_context* ctx);
+int xfmm_P2P(xfmm_context* ctx);
+int xfmm_M2L(xfmm_context* ctx);
+int xfmm_L2L(xfmm_context* ctx);
+int xfmm_L2P(xfmm_context* ctx);
+
+/* Results: T[ntrgs][4] = (phi, dphi/dx, dphi/dy, dphi/dz) at each sorted
+ * target (Node::trg_value).  Equivalent vectors: T[nnodes][nsurf] in tree node
+ * order (Node::up_equiv / Node::dn_equiv; dn holds downward check potentials
+ * until L2P converts the leaves). */
+int xfmm_get_values(xfmm_context* ctx, double* trg_value);
+int xfmm_get_equiv(xfmm_context* ctx, double* up_equiv, double* dn_equiv);
+int xfmm_set_equiv(xfmm_context* ctx, const double* up_equiv, const double* dn_equiv);
+/* Device pointer of the target values (valid until the next set_tree). */
+int xfmm_values_device(xfmm_context* ctx, void** trg_value_device);
+int xfmm_synchronize(xfmm_context* ctx);
+
+/* Per-operator device time of the last passes in milliseconds, measured with
+ * HIP events on the context stream.  names: P2M M2M P2L M2P P2P M2L L2L L2P,
+ * plus the M2L stages FFT HAD IFFT.  Returns the number of entries written. */
+int xfmm_timings(const xfmm_context* ctx, double* ms, int cap);
+
+/* Direct summation on the device, no tree: LaplaceFmm/HelmholtzFmm/
+ * ModifiedHelmholtzFmm::gradient_P2P (laplace.h:91-164, helmholtz.h:106-208,
+ * modified_helmholtz.h:94-171) when gradient = 1, potential_P2P otherwise
+ * (1 value per target).  Accumulates (+=) into trg_value like the reference.
+ * src: [ns][3], q: T[ns], trg: [nt][3], trg_value: T[nt][gradient ? 4 : 1]. */
+int xfmm_direct(int kernel, int precision, double wavek_re, double wavek_im, const double* src,
+                const double* q, int ns, const double* trg, int nt, double* trg_value, int gradient);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,164 @@
+"""GPU parity: the gfx950 engine (through the C ABI) against the oracle.
+
+Bar (FP64 path): relative L2 difference <= 1e-10 against the oracle on the same
+tree, operators and charges, after every operator of the evaluation phase and
+for the full upward + downward pass.  Mixed path: within 1e-5 of the FP64
+oracle and of the FP64 direct sum at the sizes used here.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from exafmm import abi
+from fmm_cases import make_problem, rel_err
+
+pytestmark = pytest.mark.gpu
+
+KERNELS = [("laplace", 0, 0.0), ("modified_helmholtz", 2, 5.0), ("helmholtz", 1, 5.0 + 2.0j)]
+STAGE_ORDER = ["P2M", "M2M", "P2L", "M2P", "P2P", "M2L", "L2L", "L2P"]
+FP64_TOL = 1e-10
+
+
+def close(a, b, tol):
+    nb = np.linalg.norm(b)
+    if nb == 0:
+        return np.linalg.norm(a) == 0
+    return np.linalg.norm(a - b) <= tol * nb
+
+
+@pytest.mark.parametrize("name,kid,wk", KERNELS)
+@pytest.mark.parametrize("gradient", [0, 1])
+def test_direct_fp64(require_gpu, name, kid, wk, gradient):
+    # reference tests/p2p_*.cpp: vectorised P2P vs the scalar loop, 1e-12 in FP64
+    rng = np.random.default_rng(7 + kid)
+    src, trg = rng.random((1500, 3)), rng.random((700, 3))
+    trg[:50] = src[:50]  # coincident pairs must contribute nothing
+    q = rng.random(1500) - 0.5
+    if kid == 1:
+        q = q + 1j * (rng.random(1500) - 0.5)
+    g = abi.direct(kid, src, q, trg, wk, abi.FP64, bool(gradient))
+    r = O.direct(kid, src, q, trg, wk, bool(gradient))
+    assert rel_err(g, r) < 1e-12
+
+
+@pytest.mark.parametrize("name,kid,wk", KERNELS)
+def test_direct_fp32(require_gpu, name, kid, wk):
+    # reference tests/p2p_laplace.cpp:82-84: 1e-6 in single precision
+    rng = np.random.default_rng(11 + kid)
+    src, trg = rng.random((1200, 3)), rng.random((600, 3))
+    q = rng.random(1200) - 0.5
+    if kid == 1:
+        q = q + 1j * (rng.random(1200) - 0.5)
+    g = abi.direct(kid, src, q, trg, wk, abi.MIXED, True)
+    r = O.direct(kid, src, q, trg, wk, True)
+    assert rel_err(g[:, 0], r[:, 0]) < 1e-6
+    assert rel_err(g[:, 1:], r[:, 1:]) < 1e-6
+
+
+CASES = [
+    ("laplace", 4, "cube", 6000, 64),
+    ("laplace", 6, "plummer", 12000, 32),
+    ("laplace", 8, "clusters", 8000, 64),
+    ("modified_helmholtz", 5, "cube", 5000, 64),
+    ("modified_helmholtz", 6, "sphere", 8000, 32),
+    ("helmholtz", 5, "cube", 5000, 64),
+    ("helmholtz", 5, "clusters", 6000, 32),
+]
+
+
+def _engine(pr, precision):
+    eng = abi.Engine(pr["kid"], pr["p"], pr["wavek"], precision)
+    eng.set_operators(pr["ops"])
+    eng.set_tree(pr["flat"])
+    eng.set_charges(pr["qs"])
+    return eng
+
+
+@pytest.mark.parametrize("kernel,p,dist,n,ncrit", CASES)
+def test_operators_fp64(require_gpu, kernel, p, dist, n, ncrit):
+    pr = make_problem(kernel, n, p, ncrit=ncrit, dist=dist, seed=3)
+    print("case", kernel, p, dist, n, ncrit, "nodes", pr["flat"]["nnodes"], flush=True)
+    eng = _engine(pr, abi.FP64)
+    orc = O.Oracle(pr["kid"], p, pr["flat"], pr["ops"], pr["wavek"])
+    orc.set_charges(pr["qs"])
+    eng.clear()
+    orc.clear()
+    for st in STAGE_ORDER:
+        eng.op(st)
+        orc.run(O.STAGES[st])
+        up, dn = eng.equiv()
+        v = eng.values()
+        assert close(up, orc.up, FP64_TOL), (st, "up", rel_err(up, orc.up))
+        assert close(dn, orc.dn, FP64_TOL), (st, "dn", rel_err(dn, orc.dn))
+        assert close(v, orc.val, FP64_TOL), (st, "values", rel_err(v, orc.val))
+    # full passes (each pass zeroes what it produces): twice, identical results
+    ref = orc.evaluate()
+    for _ in range(2):
+        eng.upward_pass()
+        eng.downward_pass()
+        v = eng.values()
+        assert rel_err(v[:, 0], ref[:, 0]) < FP64_TOL
+        assert rel_err(v[:, 1:], ref[:, 1:]) < FP64_TOL
+
+
+@pytest.mark.parametrize("kernel,p,dist,n,ncrit", [CASES[0], CASES[2], CASES[3], CASES[5]])
+def test_mixed_precision(require_gpu, kernel, p, dist, n, ncrit):
+    pr = make_problem(kernel, n, p, ncrit=ncrit, dist=dist, seed=5)
+    eng = _engine(pr, abi.MIXED)
+    eng.upward_pass()
+    eng.downward_pass()
+    v = eng.values()
+    orc = O.Oracle(pr["kid"], p, pr["flat"], pr["ops"], pr["wavek"])
+    orc.set_charges(pr["qs"])
+    ref = orc.evaluate()
+    tc = pr["flat"]["trg_coord"].reshape(-1, 3)
+    idx = np.arange(0, tc.shape[0], 7)
+    d = O.direct(pr["kid"], pr["flat"]["src_coord"].reshape(-1, 3), pr["qs"], tc[idx], pr["wavek"])
+    e64 = rel_err(ref[idx, 0], d[:, 0])
+    emx = rel_err(v[idx, 0], d[:, 0])
+    print(kernel, "fp64 err", e64, "mixed err", emx)
+    assert rel_err(v[:, 0], ref[:, 0]) < 1e-5
+    assert rel_err(v[:, 1:], ref[:, 1:]) < 1e-4
+    assert emx < max(2 * e64, 1e-5)
+
+
+def test_python_api_evaluate_and_verify(require_gpu):
+    import exafmm
+    L = exafmm.laplace
+    rng = np.random.default_rng(2)
+    n = 20000
+    x = rng.random((n, 3))
+    q = rng.random(n)
+    fmm = L.LaplaceFmm(p=8, ncrit=64, filename="")
+    tree = L.setup(L.init_sources(x, q), L.init_targets(x), fmm)
+    out = L.evaluate(tree, fmm)
+    assert out.shape == (n, 4)
+    ep, eg = fmm.verify(tree.leafs, True)
+    assert ep < 1e-6 and eg < 1e-5, (ep, eg)
+    ref = O.direct(0, x, q, x[::97])
+    assert rel_err(out[::97, 0], ref[:, 0]) < 1e-6
+    # BEM-style loop: update charges, clear, evaluate again (python/exafmm.cpp:275-323)
+    q2 = rng.random(n) - 0.5
+    L.update_charges(tree, q2)
+    L.clear_values(tree)
+    out2 = L.evaluate(tree, fmm)
+    ref2 = O.direct(0, x, q2, x[::97])
+    assert rel_err(out2[::97, 0], ref2[:, 0]) < 1e-6
+
+
+def test_linearity_and_repeatability(require_gpu):
+    pr = make_problem("laplace", 30000, 6, ncrit=64, dist="cube", seed=9)
+    eng = _engine(pr, abi.FP64)
+    rng = np.random.default_rng(4)
+    q1, q2 = rng.random(pr["qs"].size) - 0.5, rng.random(pr["qs"].size) - 0.5
+    res = []
+    for q in (q1, q2, q1 + 2.0 * q2):
+        eng.set_charges(q)
+        eng.upward_pass()
+        eng.downward_pass()
+        res.append(eng.values())
+    assert rel_err(res[2], res[0] + 2.0 * res[1]) < 1e-12
+    eng.set_charges(q1)
+    eng.upward_pass()
+    eng.downward_pass()
+    assert np.array_equal(eng.values(), res[0])
