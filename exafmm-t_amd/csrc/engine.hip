@@ -351,6 +351,15 @@ void op_M2L(xfmm_context* c, bool timed) {
 
 extern "C" {
 
+int xfmm_device_count(int* count) {
+  if (!count) return fail(nullptr, XFMM_ERR_ARG, "xfmm_device_count: null output");
+  *count = 0;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return XFMM_OK;
+  *count = n;
+  return XFMM_OK;
+}
+
 int xfmm_create(int kernel, int p, double wavek_re, double wavek_im, int precision, int device, xfmm_context** out) {
   if (!out) return fail(nullptr, XFMM_ERR_ARG, "xfmm_create: null output pointer");
   *out = nullptr;

@@ -47,6 +47,7 @@ def load(path=LIB_PATH):
         lib = ctypes.CDLL(path)
         vp = ctypes.c_void_p
         sig = {
+            "xfmm_device_count": ([_c_int_p], ctypes.c_int),
             "xfmm_create": ([ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                              ctypes.c_int, ctypes.POINTER(vp)], ctypes.c_int),
             "xfmm_destroy": ([vp], None),
@@ -80,6 +81,17 @@ def load(path=LIB_PATH):
 
 class XfmmError(RuntimeError):
     pass
+
+
+def device_count():
+    """HIP devices visible to the engine (0 without a GPU or without the library)."""
+    try:
+        lib = load()
+    except OSError:
+        return 0
+    n = ctypes.c_int(0)
+    lib.xfmm_device_count(ctypes.byref(n))
+    return n.value
 
 
 def _ptr(a):

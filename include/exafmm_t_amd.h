@@ -39,6 +39,12 @@ enum {
 
 typedef struct xfmm_context xfmm_context;
 
+/* Number of HIP devices visible to the engine's HIP runtime (0 if none).
+ * The engine links the system ROCm runtime; a process that has already
+ * initialised another HIP runtime copy (for example one bundled with a
+ * Python wheel) sees no device here. */
+int xfmm_device_count(int* count);
+
 /* Flattened octree.  Node order is level-major: nodes of level l are
  * [level_offset[l], level_offset[l+1]).  The 8 children of a non-leaf n are
  * the consecutive nodes child0[n] .. child0[n]+7 in octant order (x bit 0,

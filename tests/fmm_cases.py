@@ -1,7 +1,15 @@
 """Seeded problem generators shared by the tests (host side only)."""
+import os
+import tempfile
+
 import numpy as np
 
 import exafmm
+
+# precompute cache files (reference layout) go to a private temp directory,
+# keyed by kernel, p and wave number (the file format itself does not record
+# the wave number)
+CACHE = tempfile.mkdtemp(prefix="exafmm_ops_")
 
 KERNELS = {
     "laplace": (exafmm.laplace, "LaplaceFmm", 0, None),
@@ -41,7 +49,8 @@ def make_problem(kernel, n, p, ncrit=64, dist="cube", seed=0, wavek=None, precis
     src = mod.init_sources(x, q)
     trg = mod.init_targets(x)
     Cls = getattr(mod, cls)
-    fmm = Cls(p, ncrit, filename="") if wk is None else Cls(p, ncrit, wk, filename="")
+    fname = os.path.join(CACHE, "%s_p%d_k%s_%d.dat" % (kernel, p, str(wk).replace(" ", ""), seed))
+    fmm = Cls(p, ncrit, filename=fname) if wk is None else Cls(p, ncrit, wk, filename=fname)
     fmm.precision = precision
     tree = mod.setup(src, trg, fmm)
     return dict(mod=mod, fmm=fmm, tree=tree, kid=kid, wavek=0.0 if wk is None else wk, x=x, q=q, p=p,
@@ -50,3 +59,40 @@ def make_problem(kernel, n, p, ncrit=64, dist="cube", seed=0, wavek=None, precis
 
 def rel_err(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def kernel_np(kid, wavek, d):
+    """Unit-charge potential 1/(4 pi r) (0), e^{ikr}/(4 pi r) (1), e^{-kr}/(4 pi r) (2)."""
+    r = np.sqrt(np.sum(d * d, axis=-1))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        if kid == 0:
+            v = 1.0 / r
+        elif kid == 2:
+            v = np.exp(-np.real(wavek) * r) / r
+        else:
+            v = np.exp(1j * complex(wavek) * r) / r
+    v = np.where(r > 0, v, 0.0)
+    return v / (4 * np.pi)
+
+
+def check_matrix(kid, wavek, p, r0, level, up):
+    """Kernel matrix from an equivalent surface to its check surface at `level`:
+    upward (equivalent 1.05 -> check 2.95) or downward (2.95 -> 1.05)."""
+    import oracle as O
+    c = np.zeros(3)
+    a_chk, a_eq = (2.95, 1.05) if up else (1.05, 2.95)
+    chk = O.surface(p, r0, level, c, a_chk)
+    eq = O.surface(p, r0, level, c, a_eq)
+    return kernel_np(kid, wavek, chk[:, None, :] - eq[None, :, :])
+
+
+def equiv_rel_diff(kid, wavek, p, r0, levels, a, b, rows, up):
+    """Relative difference of equivalent densities a, b (rows `rows`) measured by
+    the check potentials they generate: |K (a-b)| / |K b|."""
+    num = den = 0.0
+    for lv in np.unique(levels[rows]):
+        r = rows[levels[rows] == lv]
+        K = check_matrix(kid, wavek, p, r0, int(lv), up)
+        num += np.linalg.norm((a[r] - b[r]) @ K.T) ** 2
+        den += np.linalg.norm(b[r] @ K.T) ** 2
+    return float(np.sqrt(num / den)) if den > 0 else float(np.sqrt(num))

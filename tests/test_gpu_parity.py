@@ -10,7 +10,7 @@ import pytest
 
 import oracle as O
 from exafmm import abi
-from fmm_cases import make_problem, rel_err
+from fmm_cases import equiv_rel_diff, make_problem, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -81,6 +81,12 @@ def test_operators_fp64(require_gpu, kernel, p, dist, n, ncrit):
     eng = _engine(pr, abi.FP64)
     orc = O.Oracle(pr["kid"], p, pr["flat"], pr["ops"], pr["wavek"])
     orc.set_charges(pr["qs"])
+    f = pr["flat"]
+    lev = f["level"]
+    leaf = f["child0"] < 0
+    conv = np.nonzero(leaf & ((f["src_count"] > 0) | (f["trg_count"] > 0)))[0]  # leaves L2P converts
+    other = np.setdiff1d(np.arange(f["nnodes"]), conv)
+    allrows = np.arange(f["nnodes"])
     eng.clear()
     orc.clear()
     for st in STAGE_ORDER:
@@ -88,8 +94,17 @@ def test_operators_fp64(require_gpu, kernel, p, dist, n, ncrit):
         orc.run(O.STAGES[st])
         up, dn = eng.equiv()
         v = eng.values()
-        assert close(up, orc.up, FP64_TOL), (st, "up", rel_err(up, orc.up))
-        assert close(dn, orc.dn, FP64_TOL), (st, "dn", rel_err(dn, orc.dn))
+        # equivalent densities come out of the ill-conditioned pseudo-inverse:
+        # compare the check potentials they generate (raw difference printed)
+        e_up = equiv_rel_diff(pr["kid"], pr["wavek"], p, f["r0"], lev, up, orc.up, allrows, True)
+        print(st, "up raw %.2e check-metric %.2e" % (rel_err(up, orc.up) if np.any(orc.up) else 0, e_up))
+        assert e_up <= FP64_TOL, (st, "up", e_up)
+        if st == "L2P":
+            e_dn = equiv_rel_diff(pr["kid"], pr["wavek"], p, f["r0"], lev, dn, orc.dn, conv, False)
+            assert e_dn <= FP64_TOL, (st, "dn equiv", e_dn)
+            assert close(dn[other], orc.dn[other], FP64_TOL), (st, "dn", rel_err(dn[other], orc.dn[other]))
+        else:
+            assert close(dn, orc.dn, FP64_TOL), (st, "dn", rel_err(dn, orc.dn))
         assert close(v, orc.val, FP64_TOL), (st, "values", rel_err(v, orc.val))
     # full passes (each pass zeroes what it produces): twice, identical results
     ref = orc.evaluate()
@@ -129,7 +144,9 @@ def test_python_api_evaluate_and_verify(require_gpu):
     n = 20000
     x = rng.random((n, 3))
     q = rng.random(n)
-    fmm = L.LaplaceFmm(p=8, ncrit=64, filename="")
+    import os
+    from fmm_cases import CACHE
+    fmm = L.LaplaceFmm(p=8, ncrit=64, filename=os.path.join(CACHE, "api_laplace_p8.dat"))
     tree = L.setup(L.init_sources(x, q), L.init_targets(x), fmm)
     out = L.evaluate(tree, fmm)
     assert out.shape == (n, 4)
