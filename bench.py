@@ -1,0 +1,236 @@
+"""Benchmark: Laplace FMM evaluation, N=2M uniform, p=10, ncrit=128 on one MI355X.
+
+One step = upward_pass + downward_pass of the whole tree with charges already
+resident in HBM (the FMM evaluation phase; the host tree/list/precompute setup
+is reported separately).  The headline `value` is the mixed-precision path
+(FP32 P2P and M2L, FP64 pseudo-inverse solves and translations); the plain
+FP64 path is timed beside it.  Errors are relative L2 against an FP64 direct
+sum on 2,000 seeded sample targets computed by the engine's direct-sum kernel.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--n 2000000] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "exafmm-t_amd"))
+
+METRIC = "particles/s, Laplace FMM N=2M at err≤1e-5: mixed-precision vs FP64 on GPU"
+
+# algorithmic counts (fixed before measurement)
+P2P_FLOP_PER_PAIR = 20          # Laplace potential + gradient, rsqrt = 1
+HAD_FLOP_PER_PAIR_FREQ = 512    # 8x8 complex matrix x 8-vector per (src, trg) pair per frequency
+PEAK = {"fp32_vector": 157.3e12, "fp64_vector": 78.6e12, "fp32_mfma": 157.3e12, "fp64_mfma": 78.6e12,
+        "hbm": 8.0e12}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def setup_problem(n, p, ncrit, seed):
+    import exafmm
+    L = exafmm.laplace
+    rng = np.random.default_rng(seed)
+    x = rng.random((n, 3))
+    q = rng.uniform(-1.0, 1.0, n)
+    fmm = L.LaplaceFmm(p, ncrit, os.path.join(tempfile.gettempdir(), "bench_laplace_p%d.dat" % p))
+    t0 = time.perf_counter()
+    tree = L.setup(L.init_sources(x, q), L.init_targets(x), fmm)
+    t_setup = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    flat = fmm.flat_tree(tree)
+    ops = fmm.operators()
+    qs = fmm.sorted_charges(tree)
+    t_flat = time.perf_counter() - t0
+    return dict(x=x, q=q, flat=flat, ops=ops, qs=qs, depth=fmm.depth, t_setup=t_setup, t_flat=t_flat)
+
+
+def p2p_pairs(flat):
+    off, src = flat["p2p_offset"], flat["p2p_source"]
+    cnt = flat["src_count"]
+    per_target = np.add.reduceat(cnt[src], off[:-1]) if src.size else np.zeros(flat["nnodes"])
+    per_target = np.where(off[1:] > off[:-1], per_target, 0)
+    return int(np.sum(per_target.astype(np.int64) * flat["trg_count"]))
+
+
+def run_path(prob, p, precision, steps, warmup):
+    from exafmm import abi
+    eng = abi.Engine(abi.LAPLACE, p, 0.0, precision, 0)
+    eng.set_operators(prob["ops"])
+    eng.set_tree(prob["flat"])
+    eng.set_charges(prob["qs"])
+    eng.synchronize()
+    for _ in range(warmup):
+        eng.upward_pass()
+        eng.downward_pass()
+    eng.synchronize()
+    per_step = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ts = time.perf_counter()
+        eng.upward_pass()
+        eng.downward_pass()
+        eng.synchronize()
+        per_step.append(time.perf_counter() - ts)
+    total = time.perf_counter() - t0
+    kt = eng.timings()
+    vals = eng.values()
+    eng.close()
+    return dict(total=total, per_step=per_step, kernels=kt, values=vals)
+
+
+def sample_error(prob, values, nsample, seed):
+    from exafmm import abi
+    flat = prob["flat"]
+    tc = flat["trg_coord"].reshape(-1, 3)
+    rng = np.random.default_rng(seed)
+    idx = np.sort(rng.choice(tc.shape[0], size=min(nsample, tc.shape[0]), replace=False))
+    ref = abi.direct(abi.LAPLACE, flat["src_coord"].reshape(-1, 3), prob["qs"], tc[idx], 0.0, abi.FP64, True)
+    v = values[idx]
+    ep = float(np.linalg.norm(v[:, 0] - ref[:, 0]) / np.linalg.norm(ref[:, 0]))
+    eg = float(np.linalg.norm(v[:, 1:] - ref[:, 1:]) / np.linalg.norm(ref[:, 1:]))
+    return ep, eg
+
+
+def cpu_baseline(p, ncrit, n_sample):
+    """The oracle (plain C + OpenMP restatement of the reference evaluation)
+    timed on the host cores on a bounded sample: the same workload at a
+    smaller N (same distribution, p, ncrit)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    prob = setup_problem(n_sample, p, ncrit, seed=1)
+    orc = O.Oracle(O.LAPLACE, p, prob["flat"], prob["ops"])
+    orc.set_charges(prob["qs"])
+    t0 = time.perf_counter()
+    orc.evaluate()
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": n_sample / dt, "unit": "particles/s", "cores": cores, "kind": "port",
+            "sample": "oracle full upward+downward pass, N=%d uniform (same p=%d, ncrit=%d), %.1f s"
+                      % (n_sample, p, ncrit, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=2_000_000)
+    ap.add_argument("--p", type=int, default=10)
+    ap.add_argument("--ncrit", type=int, default=128)
+    ap.add_argument("--nsample", type=int, default=2000)
+    ap.add_argument("--cpu-sample", type=int, default=100_000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--fp64-only", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build evaluates one FMM on one GPU; multi-GPU runs are not part of it "
+              "(requested --gpus %d)" % a.gpus, file=sys.stderr)
+        sys.exit(2)
+    from exafmm import abi
+    if abi.device_count() < 1:
+        print("bench.py: no GPU visible to the engine", file=sys.stderr)
+        sys.exit(1)
+
+    log("setup: N=%d p=%d ncrit=%d" % (a.n, a.p, a.ncrit))
+    prob = setup_problem(a.n, a.p, a.ncrit, seed=0)
+    flat = prob["flat"]
+    log("setup %.2fs flatten %.2fs depth %d nodes %d" % (prob["t_setup"], prob["t_flat"], prob["depth"],
+                                                          flat["nnodes"]))
+    npairs = p2p_pairs(flat)
+    m2l_pairs = int(np.sum(flat["m2l_source"] >= 0))
+    n1 = 2 * a.p
+    nfreq = n1 * n1 * (n1 // 2 + 1)
+
+    res = {}
+    paths = [("fp64", abi.FP64)] if a.fp64_only else [("mixed", abi.MIXED), ("fp64", abi.FP64)]
+    for name, prec in paths:
+        log("timing %s path: warmup %d steps %d" % (name, a.warmup, a.steps))
+        r = run_path(prob, a.p, prec, a.steps, a.warmup)
+        r["err"] = sample_error(prob, r["values"], a.nsample, seed=123)
+        log("%s: %.3f ms/step err phi %.2e grad %.2e kernels %s" % (
+            name, 1e3 * r["total"] / a.steps, r["err"][0], r["err"][1],
+            {k: round(v, 3) for k, v in r["kernels"].items()}))
+        del r["values"]
+        res[name] = r
+
+    head = res["mixed"] if "mixed" in res else res["fp64"]
+    ms_step = 1e3 * head["total"] / a.steps
+    value = a.n / (head["total"] / a.steps)
+    kt = head["kernels"]
+    # dominant kernel and its roofline
+    cand = {"P2P": kt.get("P2P", 0.0), "HAD": kt.get("HAD", 0.0), "FFT": kt.get("FFT", 0.0),
+            "IFFT": kt.get("IFFT", 0.0)}
+    dom = max(cand, key=cand.get)
+    mixed = "mixed" in res
+    if dom == "P2P":
+        flops = P2P_FLOP_PER_PAIR * npairs
+        roof = {"kernel": "P2P near field (k_eval_targets, %s)" % ("FP32" if mixed else "FP64"),
+                "bound": "valu", "achieved": flops / (kt["P2P"] * 1e-3) / 1e12,
+                "peak": (PEAK["fp32_vector"] if mixed else PEAK["fp64_vector"]) / 1e12, "unit": "TFLOP/s",
+                "traffic": None, "algorithmic": "%d pairs x %d flop" % (npairs, P2P_FLOP_PER_PAIR)}
+    elif dom == "HAD":
+        flops = HAD_FLOP_PER_PAIR_FREQ * m2l_pairs * nfreq
+        roof = {"kernel": "M2L hadamard_product (k_hadamard, %s)" % ("FP32" if mixed else "FP64"),
+                "bound": "mfma", "achieved": flops / (kt["HAD"] * 1e-3) / 1e12,
+                "peak": (PEAK["fp32_mfma"] if mixed else PEAK["fp64_mfma"]) / 1e12, "unit": "TFLOP/s",
+                "traffic": None, "algorithmic": "%d pairs x %d freq x %d flop" % (m2l_pairs, nfreq,
+                                                                                   HAD_FLOP_PER_PAIR_FREQ)}
+    else:
+        rb = 4 if mixed else 8
+        nsrc = int(np.unique(flat["m2l_source"][flat["m2l_source"] >= 0]).size)
+        ntrg = int(np.sum(np.any(flat["m2l_source"].reshape(-1, 26) >= 0, axis=1)))
+        nslots = nsrc if dom == "FFT" else ntrg
+        byts = nslots * nfreq * 8 * 2 * rb
+        roof = {"kernel": "M2L %s" % dom, "bound": "hbm", "achieved": byts / (kt[dom] * 1e-3) / 1e9,
+                "peak": PEAK["hbm"] / 1e9, "unit": "GB/s", "traffic": None,
+                "algorithmic": "%d nodes x %d freq x 8 children complex" % (nslots, nfreq)}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["ms"] = cand[dom]
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 P2P/M2L + f64 pinv solves" if mixed else "f64",
+        "data": "synthetic (seeded uniform [0,1]^3, charges U[-1,1])",
+        "config": {"workload": "laplace_fmm_N%d_p%d_ncrit%d_uniform" % (a.n, a.p, a.ncrit), "kernel": "laplace",
+                   "N": a.n, "p": a.p, "ncrit": a.ncrit, "depth": prob["depth"], "nodes": int(flat["nnodes"]),
+                   "p2p_pairs": npairs, "m2l_pairs": m2l_pairs, "precision": "mixed" if mixed else "fp64"},
+        "err_phi": head["err"][0],
+        "err_grad": head["err"][1],
+        "kernels_ms": {k: round(v, 4) for k, v in kt.items()},
+        "setup_s": round(prob["t_setup"] + prob["t_flat"], 3),
+        "roofline": roof,
+    }
+    if mixed and "fp64" in res:
+        f = res["fp64"]
+        line["fp64_value"] = a.n / (f["total"] / a.steps)
+        line["fp64_ms_per_step"] = 1e3 * f["total"] / a.steps
+        line["fp64_err_phi"], line["fp64_err_grad"] = f["err"]
+        line["fp64_kernels_ms"] = {k: round(v, 4) for k, v in f["kernels"].items()}
+        line["mixed_vs_fp64"] = line["value"] / line["fp64_value"]
+    if not a.no_cpu_baseline:
+        log("cpu baseline (oracle) on N=%d" % a.cpu_sample)
+        line["cpu_baseline"] = cpu_baseline(a.p, a.ncrit, a.cpu_sample)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

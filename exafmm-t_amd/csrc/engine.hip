@@ -331,8 +331,11 @@ void op_M2L(xfmm_context* c, bool timed) {
   launch_fft_up(m, t, c->m2l_src_node.ptr, c->n_m2l_src, c->stream);
   if (timed) record(c, 17);
   const size_t opsz = size_t(NM2L) * c->nfreq * 64 * 2;
-  for (int l = 0; l < c->depth; l++) {
-    const int b = c->blk_level_off[l], e = c->blk_level_off[l + 1];
+  // one operator set (Laplace): a single launch over the blocks of all levels
+  const int nlaunch = (c->nm2l == 1) ? 1 : c->depth;
+  for (int l = 0; l < nlaunch; l++) {
+    const int b = (c->nm2l == 1) ? 0 : c->blk_level_off[l];
+    const int e = (c->nm2l == 1) ? c->n_m2l_blk : c->blk_level_off[l + 1];
     if (e <= b) continue;
     const int L = (c->nm2l == 1) ? 0 : l;
     const void* ops = (c->prec == XFMM_MIXED) ? static_cast<const void*>(c->m2l32.ptr + L * opsz)
