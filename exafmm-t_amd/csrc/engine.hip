@@ -99,7 +99,8 @@ struct xfmm_context {
   DevBuf<long> nl_rows, nl_child_rows;                    // node*nsurf, child0*nsurf
   // M2L
   int n_m2l_src = 0, n_m2l_trg = 0, n_m2l_blk = 0;
-  DevBuf<int> m2l_src_node, m2l_trg_node, m2l_blk_src, m2l_blk_trg;
+  DevBuf<int> m2l_src_node, m2l_trg_node, m2l_blk_trg, m2l_blk_uni, m2l_blk_nu;
+  DevBuf<short> m2l_blk_loc;
   DevBuf<double> m2l_scale;
   std::vector<int> blk_level_off;                         // [depth+2] block ranges per target level
   DevBuf<char> fft_in, fft_out;
@@ -340,8 +341,13 @@ void op_M2L(xfmm_context* c, bool timed) {
     const int L = (c->nm2l == 1) ? 0 : l;
     const void* ops = (c->prec == XFMM_MIXED) ? static_cast<const void*>(c->m2l32.ptr + L * opsz)
                                               : static_cast<const void*>(c->m2l64.ptr + L * opsz);
-    launch_hadamard(m, ops, c->m2l_blk_src.ptr + size_t(b) * NM2L * 64, c->m2l_blk_trg.ptr + size_t(b) * 64, e - b,
-                    c->stream);
+    HadBlocks hb;
+    hb.nblk = e - b;
+    hb.trg = c->m2l_blk_trg.ptr + size_t(b) * 64;
+    hb.uni = c->m2l_blk_uni.ptr + size_t(b) * HAD_UCAP;
+    hb.nunion = c->m2l_blk_nu.ptr + b;
+    hb.loc = c->m2l_blk_loc.ptr + size_t(b) * NM2L * 64;
+    launch_hadamard(m, ops, hb, c->stream);
   }
   if (timed) record(c, 18);
   launch_ifft_dn(m, t, c->m2l_trg_node.ptr, c->m2l_scale.ptr, c->n_m2l_trg, c->stream);
@@ -659,28 +665,57 @@ int xfmm_set_tree(xfmm_context* c, const xfmm_tree* t) {
         src_slot[i] = int(srcs.size());
         srcs.push_back(i);
       }
-    std::vector<int> bsrc, btrg;
+    // blocks: consecutive targets of one level (Hilbert order), at most 64
+    // targets and HAD_UCAP distinct sources each
+    std::vector<int> btrg, buni, bnu;
+    std::vector<short> bloc;
     c->blk_level_off.assign(D + 2, 0);
     int nblk = 0;
+    std::vector<int> lidx(std::max<size_t>(1, srcs.size()), -1);  // src slot -> index in current union
+    std::vector<int> cur_t, cur_u;
+    auto flush = [&]() {
+      if (cur_t.empty()) return;
+      const size_t bt = btrg.size(), bu = buni.size(), bl = bloc.size();
+      btrg.resize(bt + 64, -1);
+      buni.resize(bu + HAD_UCAP, 0);
+      bloc.resize(bl + size_t(NM2L) * 64, short(-1));
+      for (size_t u = 0; u < cur_u.size(); u++) buni[bu + u] = cur_u[u];
+      for (size_t jt = 0; jt < cur_t.size(); jt++) {
+        const int node = cur_t[jt];
+        btrg[bt + jt] = trg_slot[node];
+        for (int r = 0; r < NM2L; r++) {
+          const int sn = t->m2l_source[size_t(node) * NM2L + r];
+          if (sn >= 0) bloc[bl + size_t(r) * 64 + jt] = short(lidx[src_slot[sn]]);
+        }
+      }
+      for (int u : cur_u) lidx[u] = -1;
+      bnu.push_back(int(cur_u.size()));
+      nblk++;
+      cur_t.clear();
+      cur_u.clear();
+    };
     for (int l = 0; l <= D; l++) {
       c->blk_level_off[l] = nblk;
-      std::vector<int> lt;
-      for (int i = lo[l]; i < lo[l + 1]; i++)
-        if (trg_slot[i] >= 0) lt.push_back(i);
-      for (size_t b0 = 0; b0 < lt.size(); b0 += 64) {
-        const size_t base_s = bsrc.size(), base_t = btrg.size();
-        bsrc.resize(base_s + size_t(NM2L) * 64, -1);
-        btrg.resize(base_t + 64, -1);
-        for (size_t j = 0; j < 64 && b0 + j < lt.size(); j++) {
-          const int node = lt[b0 + j];
-          btrg[base_t + j] = trg_slot[node];
-          for (int r = 0; r < NM2L; r++) {
-            const int s = t->m2l_source[size_t(node) * NM2L + r];
-            if (s >= 0) bsrc[base_s + size_t(r) * 64 + j] = src_slot[s];
+      for (int i = lo[l]; i < lo[l + 1]; i++) {
+        if (trg_slot[i] < 0) continue;
+        int extra = 0;
+        for (int r = 0; r < NM2L; r++) {
+          const int sn = t->m2l_source[size_t(i) * NM2L + r];
+          if (sn >= 0 && lidx[src_slot[sn]] < 0) extra++;  // (a source repeated over r is counted twice: conservative)
+        }
+        if (cur_t.size() == 64 || int(cur_u.size()) + extra > HAD_UCAP) flush();
+        cur_t.push_back(i);
+        for (int r = 0; r < NM2L; r++) {
+          const int sn = t->m2l_source[size_t(i) * NM2L + r];
+          if (sn < 0) continue;
+          const int ss = src_slot[sn];
+          if (lidx[ss] < 0) {
+            lidx[ss] = int(cur_u.size());
+            cur_u.push_back(ss);
           }
         }
-        nblk++;
       }
+      flush();
     }
     c->blk_level_off[D + 1] = nblk;
     c->n_m2l_src = int(srcs.size());
@@ -689,8 +724,10 @@ int xfmm_set_tree(xfmm_context* c, const xfmm_tree* t) {
     c->m2l_src_node.upload(srcs, st);
     c->m2l_trg_node.upload(trgs, st);
     c->m2l_scale.upload(scale, st);
-    c->m2l_blk_src.upload(bsrc, st);
     c->m2l_blk_trg.upload(btrg, st);
+    c->m2l_blk_uni.upload(buni, st);
+    c->m2l_blk_nu.upload(bnu, st);
+    c->m2l_blk_loc.upload(bloc, st);
     const size_t rb = (c->prec == XFMM_MIXED) ? 4 : 8;
     const size_t per = size_t(c->nfreq) * NCHILD * 2 * rb;
     c->fft_in.alloc(per * std::max(1, c->n_m2l_src));

@@ -76,10 +76,21 @@ struct M2LDev {
 // fft_up_equiv: one (source slot, child) per work item; src_node[slot] gives
 // the node whose 8 children are transformed.
 void launch_fft_up(const M2LDev& m, const DevTree& t, const int* src_node, int nslots, hipStream_t s);
-// hadamard_product: targets in blocks of 64; blk_src[b][26][64] are source
-// slots (-1 = no source), blk_trg[b][64] target slots (-1 = padding).
-void launch_hadamard(const M2LDev& m, const void* m2l_ops, const int* blk_src, const int* blk_trg, int nblk,
-                     hipStream_t s);
+// hadamard_product over blocks of <= 64 M2L targets of one operator level.
+// Block b: trg[b][64] target slots (-1 = padding), uni[b][HAD_UCAP] source
+// slots of the union of the block's sources (nunion[b] used), loc[b][26][64]
+// local union index of the source of target t for relative position r (-1 =
+// none).
+constexpr int HAD_UCAP = 256;  // max sources per block (LDS staging)
+constexpr int HAD_KCH = 16;    // frequencies per workgroup
+struct HadBlocks {
+  int nblk = 0;
+  const int* trg = nullptr;
+  const int* uni = nullptr;
+  const int* nunion = nullptr;
+  const short* loc = nullptr;
+};
+void launch_hadamard(const M2LDev& m, const void* m2l_ops, const HadBlocks& hb, hipStream_t s);
 // ifft_dn_check: one (target slot, child) per work item, adds
 // scale[slot] * result to the child's dn vector.
 void launch_ifft_dn(const M2LDev& m, const DevTree& t, const int* trg_node, const double* scale, int nslots,

@@ -186,54 +186,112 @@ __global__ __launch_bounds__(256) void k_ifft_dn(M2LDev m, DevTree t, const int*
   }
 }
 
-constexpr int HAD_KPW = 4;               // frequencies per wave
-constexpr int HAD_KPB = 4 * HAD_KPW;     // frequencies per workgroup
+// ---------------------------------------------------------------- Hadamard
+//
+// Per frequency k the product is a complex 8x8 times 8-vector for every
+// (source, target) pair.  Written over the reals, [Re out; Im out] (16) =
+// A_rk (16x16) [Re in; Im in] (16) with A = [[Re M^T, -Im M^T], [Im M^T, Re M^T]],
+// so a tile of 16 targets is a 16 x 16 x (16 x 26) MFMA product (4 MFMA k-steps
+// per relative position).  The A fragment of a lane needs M[j2][j1] for
+// j2 in {g, g+4} (g = lane/16, j1 = lane%8); the B fragment needs the input
+// children g and g+4 of the lane's target source.
+//
+// Workgroup = one block of <= 64 M2L targets (4 waves x 16-target tiles) and
+// HAD_KCH frequencies.  The union of the block's M2L sources (<= HAD_UCAP) is
+// staged in LDS one frequency at a time, so each source vector is read from
+// HBM once per block and reused by every target of the block that needs it.
+// Grid order is XCD aware: the workgroups of one frequency chunk run on the
+// same XCD, so that chunk's operator slice is fetched into one L2 once and
+// reused by all target blocks.
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_hadamard(M2LDev m, const R* __restrict__ ops, const int* __restrict__ blk_src,
-                                                  const int* __restrict__ blk_trg) {
-  __shared__ int src_s[NM2L * 64];
-  __shared__ unsigned long long mask_s[NM2L];
-  const int b = blockIdx.y;
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  for (int e = threadIdx.x; e < NM2L * 64; e += 256) src_s[e] = blk_src[size_t(b) * NM2L * 64 + e];
-  __syncthreads();
-  if (threadIdx.x < NM2L) {
-    unsigned long long msk = 0;
-    for (int l = 0; l < 64; l++) if (src_s[threadIdx.x * 64 + l] >= 0) msk |= 1ull << l;
-    mask_s[threadIdx.x] = msk;
+struct MfmaT;
+template <>
+struct MfmaT<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  // row of result register r of lane group g (v_mfma_f32_16x16x4_f32)
+  static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
+};
+template <>
+struct MfmaT<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // row of result register r of lane group g (v_mfma_f64_16x16x4_f64)
+  static __device__ __forceinline__ int row(int g, int r) { return g + 4 * r; }
+};
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_hadamard(M2LDev m, const R* __restrict__ ops, HadBlocks hb, int nchunk) {
+  typedef typename MfmaT<R>::v4 v4;
+  __shared__ C2<R> U[HAD_UCAP * NCHILD];
+  __shared__ short loc[NM2L * 64];
+  __shared__ unsigned tile_mask[4];
+  // XCD-aware (chunk, block) mapping: XCD x = w % 8 owns chunks x, x+8, ...
+  const int w = blockIdx.x;
+  const int x = w & 7, i = w >> 3;
+  const int chunk = (i / hb.nblk) * 8 + x;
+  const int b = i % hb.nblk;
+  if (chunk >= nchunk) return;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int j = lane & 15, g = lane >> 4;
+  const int nu = hb.nunion[b];
+  const int* uni = hb.uni + size_t(b) * HAD_UCAP;
+  for (int e = tid; e < NM2L * 64; e += 256) loc[e] = hb.loc[size_t(b) * NM2L * 64 + e];
   __syncthreads();
-  const int trg = blk_trg[size_t(b) * 64 + lane];
-  const C2<R>* in_base = reinterpret_cast<const C2<R>*>(m.fft_in);
-  C2<R>* out_base = reinterpret_cast<C2<R>*>(m.fft_out);
-  for (int j = 0; j < HAD_KPW; j++) {
-    const int k = blockIdx.x * HAD_KPB + wave * HAD_KPW + j;
+  if (tid < 4) {
+    unsigned msk = 0;
+    for (int r = 0; r < NM2L; r++)
+      for (int t = 0; t < 16; t++)
+        if (loc[r * 64 + tid * 16 + t] >= 0) { msk |= 1u << r; break; }
+    tile_mask[tid] = msk;
+  }
+  const int trg = hb.trg[size_t(b) * 64 + wave * 16 + j];
+  const C2<R>* fin = reinterpret_cast<const C2<R>*>(m.fft_in);
+  C2<R>* fout = reinterpret_cast<C2<R>*>(m.fft_out);
+  const C2<R>* M = reinterpret_cast<const C2<R>*>(ops);
+  const int j1 = j & 7;
+  const bool im_out = j >= 8;
+  for (int it = 0; it < HAD_KCH; it++) {
+    const int k = chunk * HAD_KCH + it;
     if (k >= m.nfreq) break;
-    C2<R> acc[NCHILD];
-#pragma unroll
-    for (int i = 0; i < NCHILD; i++) acc[i] = C2<R>{0, 0};
+    __syncthreads();
+    for (int e = tid; e < nu * NCHILD; e += 256)
+      U[e] = fin[(size_t(uni[e >> 3]) * m.nfreq + k) * NCHILD + (e & 7)];
+    __syncthreads();
+    const unsigned msk = tile_mask[wave];
+    if (msk == 0) continue;
+    v4 acc = {0, 0, 0, 0};
     for (int r = 0; r < NM2L; r++) {
-      if (mask_s[r] == 0) continue;  // no target of this block has relative position r
-      int s = src_s[r * 64 + lane];
-      const bool have = s >= 0;
-      if (!have) s = 0;
-      C2<R> v[NCHILD];
-      const C2<R>* in = in_base + (size_t(s) * m.nfreq + k) * NCHILD;
-#pragma unroll
-      for (int i = 0; i < NCHILD; i++) v[i] = have ? in[i] : C2<R>{0, 0};
-      const C2<R>* M = reinterpret_cast<const C2<R>*>(ops) + (size_t(r) * m.nfreq + k) * (NCHILD * NCHILD);
-#pragma unroll
-      for (int j2 = 0; j2 < NCHILD; j2++) {
-#pragma unroll
-        for (int j1 = 0; j1 < NCHILD; j1++) cfma(acc[j1], M[j2 * NCHILD + j1], v[j2]);
+      if (!((msk >> r) & 1u)) continue;
+      const C2<R>* Mk = M + (size_t(r) * m.nfreq + k) * (NCHILD * NCHILD);
+      const C2<R> m0 = Mk[g * NCHILD + j1];
+      const C2<R> m1 = Mk[(g + 4) * NCHILD + j1];
+      const int u = loc[r * 64 + wave * 16 + j];
+      C2<R> v0{0, 0}, v1{0, 0};
+      if (u >= 0) {
+        v0 = U[u * NCHILD + g];
+        v1 = U[u * NCHILD + g + 4];
       }
+      acc = MfmaT<R>::mma(im_out ? m0.y : m0.x, v0.x, acc);
+      acc = MfmaT<R>::mma(im_out ? m1.y : m1.x, v1.x, acc);
+      acc = MfmaT<R>::mma(im_out ? m0.x : -m0.y, v0.y, acc);
+      acc = MfmaT<R>::mma(im_out ? m1.x : -m1.y, v1.y, acc);
     }
+    // result: lane (g, j) register q holds row MfmaT::row(g, q) of target j;
+    // rows 0..7 are Re of children 0..7, rows 8..15 Im.
+    R* o = reinterpret_cast<R*>(fout + (size_t(trg < 0 ? 0 : trg) * m.nfreq + k) * NCHILD);
     if (trg >= 0) {
-      C2<R>* out = out_base + (size_t(trg) * m.nfreq + k) * NCHILD;
 #pragma unroll
-      for (int i = 0; i < NCHILD; i++) out[i] = acc[i];
+      for (int q = 0; q < 4; q++) {
+        const int rw = MfmaT<R>::row(g, q);
+        o[2 * (rw & 7) + (rw >> 3)] = acc[q];
+      }
     }
   }
 }
@@ -302,14 +360,15 @@ void launch_ifft_dn(const M2LDev& m, const DevTree& t, const int* trg_node, cons
   }
 }
 
-void launch_hadamard(const M2LDev& m, const void* m2l_ops, const int* blk_src, const int* blk_trg, int nblk,
-                     hipStream_t s) {
-  if (nblk <= 0) return;
-  dim3 grid((m.nfreq + HAD_KPB - 1) / HAD_KPB, nblk);
+void launch_hadamard(const M2LDev& m, const void* m2l_ops, const HadBlocks& hb, hipStream_t s) {
+  if (hb.nblk <= 0) return;
+  const int nchunk = (m.nfreq + HAD_KCH - 1) / HAD_KCH;
+  const int nchunk8 = (nchunk + 7) / 8 * 8;
+  dim3 grid(unsigned(nchunk8) * unsigned(hb.nblk));
   if (m.mixed)
-    hipLaunchKernelGGL((k_hadamard<float>), grid, dim3(256), 0, s, m, static_cast<const float*>(m2l_ops), blk_src, blk_trg);
+    hipLaunchKernelGGL((k_hadamard<float>), grid, dim3(256), 0, s, m, static_cast<const float*>(m2l_ops), hb, nchunk);
   else
-    hipLaunchKernelGGL((k_hadamard<double>), grid, dim3(256), 0, s, m, static_cast<const double*>(m2l_ops), blk_src, blk_trg);
+    hipLaunchKernelGGL((k_hadamard<double>), grid, dim3(256), 0, s, m, static_cast<const double*>(m2l_ops), hb, nchunk);
 }
 
 }  // namespace xfmm
