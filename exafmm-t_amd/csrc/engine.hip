@@ -105,7 +105,7 @@ struct xfmm_context {
   std::vector<int> blk_level_off;                         // [depth+2] block ranges per target level
   DevBuf<char> fft_in, fft_out;
   // buffers
-  DevBuf<double> up, dn, trg_value, check, tmp;
+  DevBuf<double> up, dn, trg_value, check, tmp, gemm_scratch;
 
   // ---- timing
   hipEvent_t ev[24] = {};
@@ -272,7 +272,8 @@ void op_M2M(xfmm_context* c) {
     const int L = c->op_level(l);
     // parent row = [child_0 .. child_7] (contiguous, K = 8 nsurf) x M2Mcat
     launch_gemm_rows(c->cplx, e - b, c->nsurf, 8 * c->nsurf, c->up.ptr, c->nl_child_rows.ptr + b,
-                     c->m2mcat.ptr + L * blk, c->up.ptr, c->nl_rows.ptr + b, nullptr, 1, c->stream);
+                     c->m2mcat.ptr + L * blk, c->up.ptr, c->nl_rows.ptr + b, nullptr, 1, c->stream,
+                     c->gemm_scratch.ptr, c->gemm_scratch.n);
   }
 }
 
@@ -283,7 +284,8 @@ void op_L2L(xfmm_context* c) {
     if (e <= b) continue;
     const int L = c->op_level(l);
     launch_gemm_rows(c->cplx, e - b, 8 * c->nsurf, c->nsurf, c->dn.ptr, c->nl_rows.ptr + b, c->l2lcat.ptr + L * blk,
-                     c->dn.ptr, c->nl_child_rows.ptr + b, nullptr, 1, c->stream);
+                     c->dn.ptr, c->nl_child_rows.ptr + b, nullptr, 1, c->stream, c->gemm_scratch.ptr,
+                     c->gemm_scratch.n);
   }
 }
 
@@ -740,6 +742,16 @@ int xfmm_set_tree(xfmm_context* c, const xfmm_tree* t) {
     c->trg_value.alloc(std::max<size_t>(1, size_t(t->ntrgs) * 4 * c->vw));
     c->check.alloc(std::max<size_t>(1, leaf.size() * c->nsurf * c->vw));
     c->tmp.alloc(std::max<size_t>(1, leaf.size() * c->nsurf * c->vw));
+    {
+      size_t need = 0;  // split-K scratch of the per-level M2M / L2L GEMMs
+      for (int l = 0; l < D; l++) {
+        const int rows = c->nl_level_off[l + 1] - c->nl_level_off[l];
+        if (rows <= 0 || c->cplx) continue;
+        need = std::max(need, gemm_scratch_doubles(rows, c->nsurf, 8 * c->nsurf));
+        need = std::max(need, gemm_scratch_doubles(rows, 8 * c->nsurf, c->nsurf));
+      }
+      c->gemm_scratch.alloc(need);
+    }
     c->q.alloc(std::max<size_t>(1, size_t(t->nsrcs) * c->vw));
     launch_zero(c->q.ptr, c->q.n, st);
     launch_zero(c->up.ptr, c->up.n, st);

@@ -10,76 +10,13 @@
 // (fmm_scale_invariant.h:260-268: N = 8*nsurf against the concatenated
 // transposed L2L matrices, accumulating into the contiguous child vectors).
 //
-// Tiling: 256 threads, 128 x 64 (real) output tile, 8 x 4 outputs per thread,
-// K staged through LDS in steps of 8.  Complex: 64 x 64 tile, 4 x 4 complex
-// outputs per thread.
+// Real operands run on FP64 MFMA (k_gemm_rows_mfma64, split-K when the output
+// has too few tiles to fill the chip).  Complex operands (Helmholtz) use a
+// VALU kernel: 256 threads, 64 x 64 tile, 4 x 4 complex outputs per thread,
+// K staged through LDS in steps of 8.
 #include "kernels.h"
 
 namespace xfmm {
-
-template <int BM, int BN, int TM, int TN, int BK>
-__global__ __launch_bounds__(256) void k_gemm_rows_real(int m, int n, int k, const double* __restrict__ A,
-                                                       const long* __restrict__ arow, const double* __restrict__ B,
-                                                       double* __restrict__ C, const long* __restrict__ crow,
-                                                       const double* __restrict__ alpha, int beta) {
-  static_assert((BM / TM) * (BN / TN) == 256, "thread layout");
-  __shared__ double As[BK][BM + 1];
-  __shared__ double Bs[BK][BN];
-  __shared__ long arow_s[BM];
-  const int tid = threadIdx.x;
-  const int row0 = blockIdx.y * BM;
-  const int col0 = blockIdx.x * BN;
-  for (int r = tid; r < BM; r += 256) arow_s[r] = (row0 + r < m) ? arow[row0 + r] : -1;
-  const int tr = tid / (BN / TN);  // thread row group
-  const int tc = tid % (BN / TN);  // thread col group
-  double acc[TM][TN];
-#pragma unroll
-  for (int i = 0; i < TM; i++)
-#pragma unroll
-    for (int j = 0; j < TN; j++) acc[i][j] = 0;
-  __syncthreads();
-  for (int k0 = 0; k0 < k; k0 += BK) {
-    // A tile: BM x BK, element e -> (row e / BK, kk e % BK)
-    for (int e = tid; e < BM * BK; e += 256) {
-      const int r = e / BK, kk = e % BK;
-      const long ar = arow_s[r];
-      As[kk][r] = (ar >= 0 && k0 + kk < k) ? A[ar + k0 + kk] : 0.0;
-    }
-    for (int e = tid; e < BK * BN; e += 256) {
-      const int kk = e / BN, c = e % BN;
-      Bs[kk][c] = (k0 + kk < k && col0 + c < n) ? B[size_t(k0 + kk) * n + col0 + c] : 0.0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < BK; kk++) {
-      double a[TM], b[TN];
-#pragma unroll
-      for (int i = 0; i < TM; i++) a[i] = As[kk][tr * TM + i];
-#pragma unroll
-      for (int j = 0; j < TN; j++) b[j] = Bs[kk][tc * TN + j];
-#pragma unroll
-      for (int i = 0; i < TM; i++)
-#pragma unroll
-        for (int j = 0; j < TN; j++) acc[i][j] = fma(a[i], b[j], acc[i][j]);
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int i = 0; i < TM; i++) {
-    const int r = row0 + tr * TM + i;
-    if (r >= m) continue;
-    const long cr = crow[r];
-    const double al = alpha ? alpha[r] : 1.0;
-#pragma unroll
-    for (int j = 0; j < TN; j++) {
-      const int c = col0 + tc * TN + j;
-      if (c >= n) continue;
-      double v = al * acc[i][j];
-      if (beta) v += C[cr + c];
-      C[cr + c] = v;
-    }
-  }
-}
 
 // complex: values are interleaved (re, im); offsets are in complex elements
 template <int BM, int BN, int TM, int TN, int BK>
@@ -150,14 +87,122 @@ __global__ __launch_bounds__(256) void k_gemm_rows_cplx(int m, int n, int k, con
   }
 }
 
+// FP64 MFMA path (v_mfma_f64_16x16x4_f64): 64 x 64 output tile per 256-thread
+// workgroup, each wave a 32 x 32 sub-tile (2 x 2 MFMA tiles), K staged through
+// LDS in steps of 16.  Optional split-K (grid.z): each split writes its partial
+// product to `partial` and k_gemm_reduce combines them in a fixed order.
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_gemm_rows_mfma64(int m, int n, int k, int kper, const double* __restrict__ A,
+                                                         const long* __restrict__ arow, const double* __restrict__ B,
+                                                         double* __restrict__ C, const long* __restrict__ crow,
+                                                         const double* __restrict__ alpha, int beta,
+                                                         double* __restrict__ partial) {
+  constexpr int BM = 64, BN = 64, BK = 16;
+  __shared__ double As[BK][BM + 2];
+  __shared__ double Bs[BK][BN + 2];
+  __shared__ long arow_s[BM];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int j = lane & 15, g = lane >> 4;
+  const int wr = w >> 1, wc = w & 1;
+  const int row0 = blockIdx.y * BM, col0 = blockIdx.x * BN;
+  const int kbeg = blockIdx.z * kper, kend = min(k, kbeg + kper);
+  if (tid < BM) arow_s[tid] = (row0 + tid < m) ? arow[row0 + tid] : -1;
+  d4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) acc[a][b] = d4{0, 0, 0, 0};
+  __syncthreads();
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int e = tid + 256 * q;
+      const int r = e >> 4, kk = e & 15;
+      const long ar = arow_s[r];
+      As[kk][r] = (ar >= 0 && k0 + kk < kend) ? A[ar + k0 + kk] : 0.0;
+      const int kb = e >> 6, cb = e & 63;
+      Bs[kb][cb] = (k0 + kb < kend && col0 + cb < n) ? B[size_t(k0 + kb) * n + col0 + cb] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < BK / 4; s++) {
+      const int kk = 4 * s + g;
+      double av[2], bv[2];
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        av[t] = As[kk][wr * 32 + t * 16 + j];
+        bv[t] = Bs[kk][wc * 32 + t * 16 + j];
+      }
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // result register q of lane (g, j): row g + 4q, column j of the 16 x 16 tile
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int r = row0 + wr * 32 + a * 16 + g + 4 * q;
+      if (r >= m) continue;
+#pragma unroll
+      for (int b = 0; b < 2; b++) {
+        const int cc = col0 + wc * 32 + b * 16 + j;
+        if (cc >= n) continue;
+        if (partial) {
+          partial[(size_t(blockIdx.z) * m + r) * n + cc] = acc[a][b][q];
+        } else {
+          const long cr = crow[r];
+          double v = (alpha ? alpha[r] : 1.0) * acc[a][b][q];
+          if (beta) v += C[cr + cc];
+          C[cr + cc] = v;
+        }
+      }
+    }
+}
+
+__global__ void k_gemm_reduce(int m, int n, int nsplit, const double* __restrict__ partial, double* __restrict__ C,
+                              const long* __restrict__ crow, const double* __restrict__ alpha, int beta) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= size_t(m) * n) return;
+  const int r = int(i / n), cc = int(i % n);
+  double s = 0;
+  for (int z = 0; z < nsplit; z++) s += partial[(size_t(z) * m + r) * n + cc];
+  const long cr = crow[r];
+  double v = (alpha ? alpha[r] : 1.0) * s;
+  if (beta) v += C[cr + cc];
+  C[cr + cc] = v;
+}
+
+size_t gemm_scratch_doubles(int m, int n, int k) {
+  const int tiles = ((n + 63) / 64) * ((m + 63) / 64);
+  int split = 1;
+  while (split < 16 && tiles * split < 1024 && k / (split * 2) >= 256) split *= 2;
+  return split > 1 ? size_t(split) * m * n : 0;
+}
+
 void launch_gemm_rows(bool cplx, int m, int n, int k, const double* A, const long* arow, const double* B,
-                      double* C, const long* crow, const double* alpha, int beta, hipStream_t s) {
+                      double* C, const long* crow, const double* alpha, int beta, hipStream_t s, double* scratch,
+                      size_t scratch_doubles) {
   if (m <= 0 || n <= 0) return;
   if (!cplx) {
-    constexpr int BM = 128, BN = 64;
-    dim3 grid((n + BN - 1) / BN, (m + BM - 1) / BM);
-    hipLaunchKernelGGL((k_gemm_rows_real<BM, BN, 8, 4, 8>), grid, dim3(256), 0, s, m, n, k, A, arow, B, C, crow,
-                       alpha, beta);
+    const int tiles = ((n + 63) / 64) * ((m + 63) / 64);
+    int split = 1;
+    while (split < 16 && tiles * split < 1024 && k / (split * 2) >= 256) split *= 2;
+    if (split > 1 && (!scratch || size_t(split) * m * n > scratch_doubles)) split = 1;
+    int kper = (k + split - 1) / split;
+    kper = (kper + 15) / 16 * 16;
+    dim3 grid((n + 63) / 64, (m + 63) / 64, split);
+    hipLaunchKernelGGL(k_gemm_rows_mfma64, grid, dim3(256), 0, s, m, n, k, kper, A, arow, B, C, crow, alpha, beta,
+                       split > 1 ? scratch : nullptr);
+    if (split > 1) {
+      const size_t tot = size_t(m) * n;
+      hipLaunchKernelGGL(k_gemm_reduce, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, s, m, n, split, scratch, C,
+                         crow, alpha, beta);
+    }
   } else {
     constexpr int BM = 64, BN = 64;
     dim3 grid((n + BN - 1) / BN, (m + BM - 1) / BM);
