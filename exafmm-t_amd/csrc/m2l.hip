@@ -20,6 +20,8 @@
 // sources overlap in L2) and 16 frequencies; each wave takes 4 frequencies,
 // lanes are targets, the 8x8 matrix M[r][k] is wave-uniform (scalar loads)
 // and the 26 relative positions are accumulated in registers.
+#include <type_traits>
+
 #include "kernels.h"
 
 namespace xfmm {
@@ -186,6 +188,171 @@ __global__ __launch_bounds__(256) void k_ifft_dn(M2LDev m, DevTree t, const int*
   }
 }
 
+// ------------------------------------------- per-node transforms (real kernels)
+//
+// One workgroup per node handles all 8 children, slab by slab in kx (the
+// halved r2c axis).  For a fixed kx the 8 children x (ky, kz) spectrum values
+// are written as contiguous 8-child groups, i.e. full cache lines of the
+// [node][freq][child] layout.  Each 1-D DFT is register blocked: a thread loads
+// its p (or 2p) inputs once and produces all outputs of the line.
+
+template <typename R, int P>
+__global__ __launch_bounds__(256) void k_fft_up_node(M2LDev m, DevTree t, const int* __restrict__ src_node,
+                                                     int nslots) {
+  extern __shared__ char smem_raw[];
+  constexpr int p = P, n1 = 2 * P, nh = P + 1, p3 = P * P * P;
+  const int slot = blockIdx.x;
+  if (slot >= nslots) return;
+  C2<R>* w = reinterpret_cast<C2<R>*>(smem_raw);  // n1
+  C2<R>* A = w + n1;                              // [c][z][y]   8 p^2
+  C2<R>* B = A + NCHILD * p * p;                  // [c][z][ky]  8 p n1
+  R* g = reinterpret_cast<R*>(B + NCHILD * p * n1);  // [c][z][y][x] 8 p^3
+  make_twiddles(w, n1, -1);
+  for (int e = threadIdx.x; e < NCHILD * p3; e += blockDim.x) g[e] = R(0);
+  __syncthreads();
+  const int node = src_node[slot];
+  const double* up = t.up + size_t(t.child0[node]) * m.nsurf;
+  for (int e = threadIdx.x; e < NCHILD * m.nsurf; e += blockDim.x) {
+    const int c = e / m.nsurf, i = e % m.nsurf;
+    g[c * p3 + m.map_up[i]] = R(up[e]);
+  }
+  __syncthreads();
+  C2<R>* out = reinterpret_cast<C2<R>*>(m.fft_in) + size_t(slot) * m.nfreq * NCHILD;
+  for (int kx = 0; kx < nh; kx++) {
+    // x pass: A[c][z][y] = sum_x g[c][z][y][x] W^(kx x)
+    for (int e = threadIdx.x; e < NCHILD * p * p; e += blockDim.x) {
+      const R* gl = g + e * p;
+      C2<R> acc{0, 0};
+#pragma unroll
+      for (int x = 0; x < p; x++) {
+        const C2<R> tw = w[(kx * x) % n1];
+        acc.x = fma(gl[x], tw.x, acc.x);
+        acc.y = fma(gl[x], tw.y, acc.y);
+      }
+      A[e] = acc;
+    }
+    __syncthreads();
+    // y pass: thread (c, z): B[c][z][ky] = sum_y A[c][z][y] W^(ky y)
+    for (int e = threadIdx.x; e < NCHILD * p; e += blockDim.x) {
+      C2<R> a[p];
+#pragma unroll
+      for (int y = 0; y < p; y++) a[y] = A[e * p + y];
+      for (int ky = 0; ky < n1; ky++) {
+        C2<R> acc{0, 0};
+#pragma unroll
+        for (int y = 0; y < p; y++) cfma(acc, a[y], w[(ky * y) % n1]);
+        B[e * n1 + ky] = acc;
+      }
+    }
+    __syncthreads();
+    // z pass: thread (c, ky) with c fastest; out[kx + nh (ky + n1 kz)][c]
+    for (int e = threadIdx.x; e < NCHILD * n1; e += blockDim.x) {
+      const int c = e & 7, ky = e >> 3;
+      C2<R> b[p];
+#pragma unroll
+      for (int z = 0; z < p; z++) b[z] = B[(c * p + z) * n1 + ky];
+      for (int kz = 0; kz < n1; kz++) {
+        C2<R> acc{0, 0};
+#pragma unroll
+        for (int z = 0; z < p; z++) cfma(acc, b[z], w[(kz * z) % n1]);
+        out[size_t(kx + nh * (ky + n1 * kz)) * NCHILD + c] = acc;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename R, int P>
+__global__ __launch_bounds__(256) void k_ifft_dn_node(M2LDev m, DevTree t, const int* __restrict__ trg_node,
+                                                      const double* __restrict__ scale, int nslots) {
+  extern __shared__ char smem_raw[];
+  constexpr int p = P, n1 = 2 * P, nh = P + 1, p3 = P * P * P;
+  const int slot = blockIdx.x;
+  if (slot >= nslots) return;
+  C2<R>* w = reinterpret_cast<C2<R>*>(smem_raw);  // n1, exp(+2 pi i m / n1)
+  C2<R>* Bp = w + n1;                             // [c][zz][ky]  8 p n1
+  C2<R>* Ap = Bp + NCHILD * p * n1;               // [c][zz][yy]  8 p^2
+  R* G = reinterpret_cast<R*>(Ap + NCHILD * p * p);  // [c][zz][yy][xx] 8 p^3
+  make_twiddles(w, n1, +1);
+  for (int e = threadIdx.x; e < NCHILD * p3; e += blockDim.x) G[e] = R(0);
+  __syncthreads();
+  const C2<R>* in = reinterpret_cast<const C2<R>*>(m.fft_out) + size_t(slot) * m.nfreq * NCHILD;
+  for (int kx = 0; kx < nh; kx++) {
+    // inverse z pass, thread (c, ky): z' = p + zz
+    for (int e = threadIdx.x; e < NCHILD * n1; e += blockDim.x) {
+      const int c = e & 7, ky = e >> 3;
+      C2<R> v[n1];
+#pragma unroll
+      for (int kz = 0; kz < n1; kz++) v[kz] = in[size_t(kx + nh * (ky + n1 * kz)) * NCHILD + c];
+      for (int zz = 0; zz < p; zz++) {
+        C2<R> acc{0, 0};
+#pragma unroll
+        for (int kz = 0; kz < n1; kz++) cfma(acc, v[kz], w[(kz * (p + zz)) % n1]);
+        Bp[(c * p + zz) * n1 + ky] = acc;
+      }
+    }
+    __syncthreads();
+    // inverse y pass, thread (c, zz): y' = p + yy
+    for (int e = threadIdx.x; e < NCHILD * p; e += blockDim.x) {
+      C2<R> v[n1];
+#pragma unroll
+      for (int ky = 0; ky < n1; ky++) v[ky] = Bp[e * n1 + ky];
+      for (int yy = 0; yy < p; yy++) {
+        C2<R> acc{0, 0};
+#pragma unroll
+        for (int ky = 0; ky < n1; ky++) cfma(acc, v[ky], w[(ky * (p + yy)) % n1]);
+        Ap[e * p + yy] = acc;
+      }
+    }
+    __syncthreads();
+    // x (c2r) accumulation: weight 1 for kx = 0 and the Nyquist plane, else 2
+    const R wt = (kx == 0 || kx == nh - 1) ? R(1) : R(2);
+    for (int e = threadIdx.x; e < NCHILD * p * p; e += blockDim.x) {
+      const C2<R> a = Ap[e];
+      R* Gl = G + e * p;
+#pragma unroll
+      for (int xx = 0; xx < p; xx++) {
+        const C2<R> tw = w[(kx * (p + xx)) % n1];
+        Gl[xx] += wt * (a.x * tw.x - a.y * tw.y);
+      }
+    }
+    __syncthreads();
+  }
+  const int node = trg_node[slot];
+  const double sc = scale[slot];
+  double* dn = t.dn + size_t(t.child0[node]) * m.nsurf;
+  for (int e = threadIdx.x; e < NCHILD * m.nsurf; e += blockDim.x) {
+    const int c = e / m.nsurf, i = e % m.nsurf;
+    dn[e] += sc * double(G[c * p3 + m.map_dn[i]]);
+  }
+}
+
+static size_t fft_node_lds(const M2LDev& m, size_t rb) {
+  const size_t p = m.p, n1 = m.n1;
+  return n1 * 2 * rb + NCHILD * p * p * 2 * rb + NCHILD * p * n1 * 2 * rb + NCHILD * p * p * p * rb;
+}
+
+// per-node transforms: real kernels, p in [3, 12], LDS within the CU's 160 KiB
+static bool fft_node_ok(const M2LDev& m) {
+  return !m.cplx && m.p >= 3 && m.p <= 12 && fft_node_lds(m, m.mixed ? 4 : 8) <= 160 * 1024;
+}
+
+template <typename F>
+static void node_dispatch(int p, F&& f) {
+  switch (p) {
+    case 3: f(std::integral_constant<int, 3>()); break;
+    case 4: f(std::integral_constant<int, 4>()); break;
+    case 5: f(std::integral_constant<int, 5>()); break;
+    case 6: f(std::integral_constant<int, 6>()); break;
+    case 7: f(std::integral_constant<int, 7>()); break;
+    case 8: f(std::integral_constant<int, 8>()); break;
+    case 9: f(std::integral_constant<int, 9>()); break;
+    case 10: f(std::integral_constant<int, 10>()); break;
+    case 11: f(std::integral_constant<int, 11>()); break;
+    default: f(std::integral_constant<int, 12>()); break;
+  }
+}
+
 // ---------------------------------------------------------------- Hadamard
 //
 // Per frequency k the product is a complex 8x8 times 8-vector for every
@@ -315,6 +482,7 @@ static size_t ifft_dn_lds(const M2LDev& m, size_t rb) {
 
 size_t m2l_lds_bytes(const M2LDev& m) {
   const size_t rb = m.mixed ? 4 : 8;
+  if (fft_node_ok(m)) return fft_node_lds(m, rb);
   const size_t a = fft_up_lds(m, rb), b = ifft_dn_lds(m, rb);
   return a > b ? a : b;
 }
@@ -331,6 +499,20 @@ static void allow_lds(K kernel, size_t bytes) {
 
 void launch_fft_up(const M2LDev& m, const DevTree& t, const int* src_node, int nslots, hipStream_t s) {
   if (nslots <= 0) return;
+  if (fft_node_ok(m)) {
+    const size_t lds = fft_node_lds(m, m.mixed ? 4 : 8);
+    node_dispatch(m.p, [&](auto P) {
+      constexpr int PP = decltype(P)::value;
+      if (m.mixed) {
+        allow_lds(k_fft_up_node<float, PP>, lds);
+        hipLaunchKernelGGL((k_fft_up_node<float, PP>), dim3(nslots), dim3(256), lds, s, m, t, src_node, nslots);
+      } else {
+        allow_lds(k_fft_up_node<double, PP>, lds);
+        hipLaunchKernelGGL((k_fft_up_node<double, PP>), dim3(nslots), dim3(256), lds, s, m, t, src_node, nslots);
+      }
+    });
+    return;
+  }
   const size_t lds = fft_up_lds(m, m.mixed ? 4 : 8);
   dim3 grid(xcd_grid(nslots));
   if (m.mixed) {
@@ -347,6 +529,22 @@ void launch_fft_up(const M2LDev& m, const DevTree& t, const int* src_node, int n
 void launch_ifft_dn(const M2LDev& m, const DevTree& t, const int* trg_node, const double* scale, int nslots,
                     hipStream_t s) {
   if (nslots <= 0) return;
+  if (fft_node_ok(m)) {
+    const size_t lds = fft_node_lds(m, m.mixed ? 4 : 8);
+    node_dispatch(m.p, [&](auto P) {
+      constexpr int PP = decltype(P)::value;
+      if (m.mixed) {
+        allow_lds(k_ifft_dn_node<float, PP>, lds);
+        hipLaunchKernelGGL((k_ifft_dn_node<float, PP>), dim3(nslots), dim3(256), lds, s, m, t, trg_node, scale,
+                           nslots);
+      } else {
+        allow_lds(k_ifft_dn_node<double, PP>, lds);
+        hipLaunchKernelGGL((k_ifft_dn_node<double, PP>), dim3(nslots), dim3(256), lds, s, m, t, trg_node, scale,
+                           nslots);
+      }
+    });
+    return;
+  }
   const size_t lds = ifft_dn_lds(m, m.mixed ? 4 : 8);
   dim3 grid(xcd_grid(nslots));
   if (m.mixed) {
