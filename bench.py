@@ -126,7 +126,7 @@ def main():
     ap.add_argument("--p", type=int, default=10)
     ap.add_argument("--ncrit", type=int, default=128)
     ap.add_argument("--nsample", type=int, default=2000)
-    ap.add_argument("--cpu-sample", type=int, default=100_000)
+    ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
     a = ap.parse_args()
