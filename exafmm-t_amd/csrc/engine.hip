@@ -70,6 +70,13 @@ struct xfmm_context {
   int nsurf = 0, nconv = 0, nfreq = 0, n1 = 0, vw = 1;
   bool cplx = false;
   hipStream_t stream = nullptr;
+  // Second stream: the near-field P2P of an evaluation runs there, into
+  // `near`, concurrently with the upward pass and M2L (it needs only the
+  // charges), and is added to the target values at the end of the downward
+  // pass.
+  hipStream_t stream2 = nullptr;
+  bool near_ready = false;   // `near` holds P2P of the current charges
+  bool near_launched = false;
   std::string err;
 
   // ---- operators
@@ -105,7 +112,7 @@ struct xfmm_context {
   std::vector<int> blk_level_off;                         // [depth+2] block ranges per target level
   DevBuf<char> fft_in, fft_out;
   // buffers
-  DevBuf<double> up, dn, trg_value, check, tmp, gemm_scratch;
+  DevBuf<double> up, dn, trg_value, check, tmp, gemm_scratch, near;
 
   // ---- timing
   hipEvent_t ev[24] = {};
@@ -399,6 +406,7 @@ int xfmm_create(int kernel, int p, double wavek_re, double wavek_im, int precisi
     if (device < 0 || device >= ndev) throw ArgError{"xfmm_create: device ordinal out of range"};
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
     bool ok = true;
     for (auto& e : c->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
     c->ev_ok = ok;
@@ -437,9 +445,11 @@ void xfmm_destroy(xfmm_context* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   delete c;  // DevBuf destructors free device memory
 }
 
@@ -563,6 +573,10 @@ int xfmm_set_tree(xfmm_context* c, const xfmm_tree* t) {
       }
 
     hipStream_t st = c->stream;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream2));
+    c->near_ready = false;
+    c->near_launched = false;
     c->have_tree = false;
     c->nnodes = nn;
     c->depth = D;
@@ -753,6 +767,7 @@ int xfmm_set_tree(xfmm_context* c, const xfmm_tree* t) {
       c->gemm_scratch.alloc(need);
     }
     c->q.alloc(std::max<size_t>(1, size_t(t->nsrcs) * c->vw));
+    c->near.alloc(c->trg_value.n);
     launch_zero(c->q.ptr, c->q.n, st);
     launch_zero(c->up.ptr, c->up.n, st);
     launch_zero(c->dn.ptr, c->dn.n, st);
@@ -767,6 +782,9 @@ int xfmm_set_charges_host(xfmm_context* c, const double* q) {
   return guarded(c, [&] {
     require(c->have_tree, "xfmm_set_charges: tree not set");
     require(q || c->nsrcs == 0, "xfmm_set_charges: null charges");
+    // a near-field P2P still reading the old charges on the second stream
+    if (c->near_launched) HIPCHK(hipStreamWaitEvent(c->stream, c->ev[20], 0));
+    c->near_ready = false;
     if (c->nsrcs) HIPCHK(hipMemcpyAsync(c->q.ptr, q, size_t(c->nsrcs) * c->vw * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   });
@@ -776,6 +794,9 @@ int xfmm_set_charges_device(xfmm_context* c, const void* q) {
   return guarded(c, [&] {
     require(c->have_tree, "xfmm_set_charges: tree not set");
     require(q || c->nsrcs == 0, "xfmm_set_charges: null charges");
+    // a near-field P2P still reading the old charges on the second stream
+    if (c->near_launched) HIPCHK(hipStreamWaitEvent(c->stream, c->ev[20], 0));
+    c->near_ready = false;
     if (c->nsrcs) HIPCHK(hipMemcpyAsync(c->q.ptr, q, size_t(c->nsrcs) * c->vw * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   });
 }
@@ -793,6 +814,22 @@ int xfmm_clear_values(xfmm_context* c) {
 int xfmm_upward_pass(xfmm_context* c) {
   return guarded(c, [&] {
     check_ready(c);
+    if (!c->near_ready && c->ev_ok) {
+      // near-field P2P on the second stream, after everything already queued
+      // on the main stream (charges)
+      HIPCHK(hipEventRecord(c->ev[23], c->stream));
+      HIPCHK(hipStreamWaitEvent(c->stream2, c->ev[23], 0));
+      launch_zero(c->near.ptr, c->near.n, c->stream2);
+      HIPCHK(hipEventRecord(c->ev[21], c->stream2));
+      DevTree t = c->dev_tree();
+      t.trg_value = c->near.ptr;
+      launch_eval_targets(c->kind, c->prec == XFMM_MIXED, MODE_P2P, t, c->wavek(), c->w_p2p.ptr, c->n_p2p,
+                          c->stream2);
+      HIPCHK(hipEventRecord(c->ev[22], c->stream2));
+      HIPCHK(hipEventRecord(c->ev[20], c->stream2));
+      c->near_ready = true;
+      c->near_launched = true;
+    }
     launch_zero(c->up.ptr, c->up.n, c->stream);
     record(c, 0);
     op_P2M(c);
@@ -813,7 +850,8 @@ int xfmm_downward_pass(xfmm_context* c) {
     record(c, 4);
     op_M2P(c);
     record(c, 5);
-    op_P2P(c);
+    const bool near = c->near_ready;
+    if (!near) op_P2P(c);
     record(c, 6);
     op_M2L(c, true);
     record(c, 7);
@@ -821,6 +859,10 @@ int xfmm_downward_pass(xfmm_context* c) {
     record(c, 8);
     op_L2P(c);
     record(c, 9);
+    if (near) {
+      HIPCHK(hipStreamWaitEvent(c->stream, c->ev[20], 0));
+      launch_axpy(c->trg_value.ptr, c->near.ptr, c->trg_value.n, c->stream);
+    }
     HIPCHK(hipGetLastError());
   });
 }
@@ -886,16 +928,22 @@ int xfmm_values_device(xfmm_context* c, void** ptr) {
 }
 
 int xfmm_synchronize(xfmm_context* c) {
-  return guarded(c, [&] { HIPCHK(hipStreamSynchronize(c->stream)); });
+  return guarded(c, [&] {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream2));
+  });
 }
 
 int xfmm_timings(const xfmm_context* cc, double* ms, int cap) {
   xfmm_context* c = const_cast<xfmm_context*>(cc);
   if (!c || !ms) return 0;
   if (!c->ev_ok) return 0;
-  if (hipStreamSynchronize(c->stream) != hipSuccess) return 0;
-  // P2M M2M P2L M2P P2P M2L L2L L2P FFT HAD IFFT
-  const int pairs[11][2] = {{0, 1}, {1, 2}, {3, 4}, {4, 5}, {5, 6}, {6, 7}, {7, 8}, {8, 9}, {16, 17}, {17, 18}, {18, 19}};
+  if (hipStreamSynchronize(c->stream) != hipSuccess || hipStreamSynchronize(c->stream2) != hipSuccess) return 0;
+  // P2M M2M P2L M2P P2P M2L L2L L2P FFT HAD IFFT; P2P from the second stream
+  // when it ran there (overlapped with the other operators)
+  const bool ov = c->near_launched;
+  const int pairs[11][2] = {{0, 1}, {1, 2}, {3, 4}, {4, 5}, {ov ? 21 : 5, ov ? 22 : 6}, {6, 7}, {7, 8}, {8, 9},
+                            {16, 17}, {17, 18}, {18, 19}};
   int n = 0;
   for (int i = 0; i < 11 && n < cap; i++) {
     float v = 0;

@@ -105,6 +105,7 @@ void launch_ifft_dn(const M2LDev& m, const DevTree& t, const int* trg_node, cons
 size_t m2l_lds_bytes(const M2LDev& m);
 
 void launch_zero(double* p, size_t n, hipStream_t s);
+void launch_axpy(double* y, const double* x, size_t n, hipStream_t s);  // y += x
 void launch_scatter_values(const double* sorted_val, const int* perm, int n, int vw, double* out, hipStream_t s);
 
 }  // namespace xfmm

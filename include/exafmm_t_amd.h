@@ -15,8 +15,9 @@
  *  - Every function returns XFMM_OK (0) or a positive status code; nothing is
  *    thrown across the ABI.  xfmm_last_error() returns the message of the last
  *    failure of that context (or of the calling thread for context creation).
- *  - One context = one HIP stream on one device.  Contexts are independent;
- *    calls on one context must not overlap.
+ *  - One context = two HIP streams on one device (the near-field P2P of an
+ *    evaluation runs on the second, concurrently with the upward pass and
+ *    M2L).  Contexts are independent; calls on one context must not overlap.
  *  - Sorted order: sources/targets in the order given by xfmm_tree (grouped by
  *    leaf); the engine never sees the caller's original body order.
  */
@@ -116,7 +117,10 @@ int xfmm_clear_values(xfmm_context* ctx);
 
 /* FmmBase::upward_pass / downward_pass (include/fmm_base.h:162-200).  Each
  * pass starts by zeroing what it produces (up_equiv, resp. dn_equiv and the
- * target values), so repeated evaluation does not need clear_values. */
+ * target values), so repeated evaluation does not need clear_values.
+ * upward_pass also starts the near-field P2P of the current charges on the
+ * second stream; downward_pass adds it to the target values (setting new
+ * charges in between discards it and downward_pass recomputes it). */
 int xfmm_upward_pass(xfmm_context* ctx);
 int xfmm_downward_pass(xfmm_context* ctx);
 
