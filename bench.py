@@ -129,6 +129,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
+    ap.add_argument("--mixed-only", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build evaluates one FMM on one GPU; multi-GPU runs are not part of it "
@@ -151,6 +152,8 @@ def main():
 
     res = {}
     paths = [("fp64", abi.FP64)] if a.fp64_only else [("mixed", abi.MIXED), ("fp64", abi.FP64)]
+    if a.mixed_only:
+        paths = [("mixed", abi.MIXED)]
     for name, prec in paths:
         log("timing %s path: warmup %d steps %d" % (name, a.warmup, a.steps))
         r = run_path(prob, a.p, prec, a.steps, a.warmup)

@@ -24,6 +24,16 @@ struct Wavek {
 
 __device__ __forceinline__ double rsqrt_pos(double r2) { return r2 > 0.0 ? rsqrt(r2) : 0.0; }
 __device__ __forceinline__ float rsqrt_pos(float r2) { return r2 > 0.0f ? rsqrtf(r2) : 0.0f; }
+
+// 1/sqrt(r2) for r2 known to be positive and finite (well-separated surfaces):
+// hardware estimate plus one second-order Newton correction,
+// y (1 + e/2 + 3e^2/8) with e = 1 - r2 y^2.
+__device__ __forceinline__ double rsqrt_sep(double r2) {
+  const double y = __builtin_amdgcn_rsq(r2);
+  const double e = fma(-r2 * y, y, 1.0);
+  return fma(y * e, fma(e, 0.375, 0.5), y);
+}
+__device__ __forceinline__ float rsqrt_sep(float r2) { return __builtin_amdgcn_rsqf(r2); }
 __device__ __forceinline__ double dexp(double x) { return exp(x); }
 __device__ __forceinline__ float dexp(float x) { return expf(x); }
 __device__ __forceinline__ void dsincos(double x, double* s, double* c) { sincos(x, s, c); }

@@ -4,6 +4,7 @@
 // kernels of p2p.hip / gemm.hip / m2l.hip on one HIP stream.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <string>
@@ -66,6 +67,9 @@ struct DevBuf {
 
 struct xfmm_context {
   int kind = 0, p = 0, prec = 0, device = 0;
+  // stages that run in FP32 (mixed path only): near-field P2P, M2L
+  // (transforms + Hadamard), and the surface evaluations P2M/L2P/M2P/P2L
+  bool f32_p2p = false, f32_m2l = false, f32_surf = false;
   double wk_re = 0, wk_im = 0;
   int nsurf = 0, nconv = 0, nfreq = 0, n1 = 0, vw = 1;
   bool cplx = false;
@@ -156,7 +160,7 @@ struct xfmm_context {
     m.nfreq = nfreq;
     m.nsurf = nsurf;
     m.cplx = cplx;
-    m.mixed = prec == XFMM_MIXED;
+    m.mixed = f32_m2l;
     m.map_up = map_up.ptr;
     m.map_dn = map_dn.ptr;
     m.fft_in = fft_in.ptr;
@@ -256,7 +260,7 @@ void check_ready(xfmm_context* c) {
 void op_P2M(xfmm_context* c) {
   // check potentials of every leaf, then up = alpha * UC2E_V (UC2E_U check)
   DevTree t = c->dev_tree();
-  launch_eval_surface(c->kind, c->prec == XFMM_MIXED, MODE_P2M, t, c->wavek(), c->w_leaf.ptr, c->n_leaf, c->check.ptr,
+  launch_eval_surface(c->kind, c->f32_surf, MODE_P2M, t, c->wavek(), c->w_leaf.ptr, c->n_leaf, c->check.ptr,
                       c->stream);
   const size_t ns2 = size_t(c->nsurf) * c->nsurf * c->vw;
   for (int l = 0; l <= c->depth; l++) {
@@ -309,24 +313,24 @@ void op_L2P(xfmm_context* c) {
                      c->stream);
   }
   DevTree t = c->dev_tree();
-  launch_eval_targets(c->kind, c->prec == XFMM_MIXED, MODE_L2P, t, c->wavek(), c->w_l2p_eval.ptr, c->n_l2p_eval,
+  launch_eval_targets(c->kind, c->f32_surf, MODE_L2P, t, c->wavek(), c->w_l2p_eval.ptr, c->n_l2p_eval,
                       c->stream);
 }
 
 void op_P2L(xfmm_context* c) {
   DevTree t = c->dev_tree();
-  launch_eval_surface(c->kind, c->prec == XFMM_MIXED, MODE_P2L, t, c->wavek(), c->w_p2l.ptr, c->n_p2l, nullptr,
+  launch_eval_surface(c->kind, c->f32_surf, MODE_P2L, t, c->wavek(), c->w_p2l.ptr, c->n_p2l, nullptr,
                       c->stream);
 }
 
 void op_M2P(xfmm_context* c) {
   DevTree t = c->dev_tree();
-  launch_eval_targets(c->kind, c->prec == XFMM_MIXED, MODE_M2P, t, c->wavek(), c->w_m2p.ptr, c->n_m2p, c->stream);
+  launch_eval_targets(c->kind, c->f32_surf, MODE_M2P, t, c->wavek(), c->w_m2p.ptr, c->n_m2p, c->stream);
 }
 
 void op_P2P(xfmm_context* c) {
   DevTree t = c->dev_tree();
-  launch_eval_targets(c->kind, c->prec == XFMM_MIXED, MODE_P2P, t, c->wavek(), c->w_p2p.ptr, c->n_p2p, c->stream);
+  launch_eval_targets(c->kind, c->f32_p2p, MODE_P2P, t, c->wavek(), c->w_p2p.ptr, c->n_p2p, c->stream);
 }
 
 void record(xfmm_context* c, int i) {
@@ -348,7 +352,7 @@ void op_M2L(xfmm_context* c, bool timed) {
     const int e = (c->nm2l == 1) ? c->n_m2l_blk : c->blk_level_off[l + 1];
     if (e <= b) continue;
     const int L = (c->nm2l == 1) ? 0 : l;
-    const void* ops = (c->prec == XFMM_MIXED) ? static_cast<const void*>(c->m2l32.ptr + L * opsz)
+    const void* ops = c->f32_m2l ? static_cast<const void*>(c->m2l32.ptr + L * opsz)
                                               : static_cast<const void*>(c->m2l64.ptr + L * opsz);
     HadBlocks hb;
     hb.nblk = e - b;
@@ -391,6 +395,15 @@ int xfmm_create(int kernel, int p, double wavek_re, double wavek_im, int precisi
   c->kind = kernel;
   c->p = p;
   c->prec = precision;
+  if (precision == XFMM_MIXED) {
+    // default mixed policy: FP32 near field and M2L; XFMM_MIXED_STAGES
+    // (comma list of p2p, m2l, surf) overrides it
+    const char* env = std::getenv("XFMM_MIXED_STAGES");
+    const std::string st = env ? env : "p2p,m2l";
+    c->f32_p2p = st.find("p2p") != std::string::npos;
+    c->f32_m2l = st.find("m2l") != std::string::npos;
+    c->f32_surf = st.find("surf") != std::string::npos;
+  }
   c->device = device;
   c->wk_re = wavek_re;
   c->wk_im = wavek_im;
@@ -505,7 +518,7 @@ int xfmm_set_operators(xfmm_context* c, int nlevels, const double* uc2e_u, const
     c->m2mcat.upload(mm, c->stream);
     c->l2lcat.upload(ll, c->stream);
     const size_t msz = size_t(nm2l) * NM2L * c->nfreq * 64 * 2;
-    if (c->prec == XFMM_MIXED) {
+    if (c->f32_m2l) {
       std::vector<float> f(msz);
       for (size_t i = 0; i < msz; i++) f[i] = float(m2l[i]);
       c->m2l32.upload(f, c->stream);
@@ -744,7 +757,7 @@ int xfmm_set_tree(xfmm_context* c, const xfmm_tree* t) {
     c->m2l_blk_uni.upload(buni, st);
     c->m2l_blk_nu.upload(bnu, st);
     c->m2l_blk_loc.upload(bloc, st);
-    const size_t rb = (c->prec == XFMM_MIXED) ? 4 : 8;
+    const size_t rb = c->f32_m2l ? 4 : 8;
     const size_t per = size_t(c->nfreq) * NCHILD * 2 * rb;
     c->fft_in.alloc(per * std::max(1, c->n_m2l_src));
     c->fft_out.alloc(per * std::max(1, c->n_m2l_trg));
@@ -823,8 +836,7 @@ int xfmm_upward_pass(xfmm_context* c) {
       HIPCHK(hipEventRecord(c->ev[21], c->stream2));
       DevTree t = c->dev_tree();
       t.trg_value = c->near.ptr;
-      launch_eval_targets(c->kind, c->prec == XFMM_MIXED, MODE_P2P, t, c->wavek(), c->w_p2p.ptr, c->n_p2p,
-                          c->stream2);
+      launch_eval_targets(c->kind, c->f32_p2p, MODE_P2P, t, c->wavek(), c->w_p2p.ptr, c->n_p2p, c->stream2);
       HIPCHK(hipEventRecord(c->ev[22], c->stream2));
       HIPCHK(hipEventRecord(c->ev[20], c->stream2));
       c->near_ready = true;

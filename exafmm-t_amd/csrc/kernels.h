@@ -41,13 +41,13 @@ enum SurfaceMode { MODE_P2M = 0, MODE_P2L = 1 };
 // particles of the P2P list), M2P (sources: upward equivalent surfaces of the
 // M2P list, alpha 1.05), L2P (source: the leaf's own downward equivalent
 // surface, alpha 2.95).  Adds grad/potential into trg_value.
-void launch_eval_targets(int kind, bool mixed, int mode, const DevTree& t, Wavek k, const int* work, int nwork,
+void launch_eval_targets(int kind, bool f32, int mode, const DevTree& t, Wavek k, const int* work, int nwork,
                          hipStream_t s);
 
 // Surface targets: P2M writes the upward check potential of each work leaf to
 // check[slot] (slot = position in work); P2L adds the potential of the P2L list
 // sources on the downward check surface (alpha 1.05) to dn[node].
-void launch_eval_surface(int kind, bool mixed, int mode, const DevTree& t, Wavek k, const int* work, int nwork,
+void launch_eval_surface(int kind, bool f32, int mode, const DevTree& t, Wavek k, const int* work, int nwork,
                          double* check, hipStream_t s);
 
 // Direct sum of ns sources onto nt targets (potential or potential+gradient);

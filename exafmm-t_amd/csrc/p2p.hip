@@ -24,10 +24,10 @@ template <int KIND>
 constexpr bool kcplx() { return KIND == HELMHOLTZ; }
 
 // potential + gradient of one pair, accumulated into a[] (4 or 8 values)
-template <int KIND, typename R>
+template <int KIND, typename R, bool SEP = false>
 __device__ __forceinline__ void pair_grad(R dx, R dy, R dz, R qr, R qi, R kr, R ki, R* a) {
   const R r2 = dx * dx + dy * dy + dz * dz;
-  const R invr = rsqrt_pos(r2);
+  const R invr = SEP ? rsqrt_sep(r2) : rsqrt_pos(r2);
   if constexpr (KIND == LAPLACE) {
     const R qinv = qr * invr;
     a[0] += qinv;
@@ -67,10 +67,10 @@ __device__ __forceinline__ void pair_grad(R dx, R dy, R dz, R qr, R qi, R kr, R 
 }
 
 // potential of one pair into a[0] (and a[1] for Helmholtz)
-template <int KIND, typename R>
+template <int KIND, typename R, bool SEP = false>
 __device__ __forceinline__ void pair_pot(R dx, R dy, R dz, R qr, R qi, R kr, R ki, R* a) {
   const R r2 = dx * dx + dy * dy + dz * dz;
-  const R invr = rsqrt_pos(r2);
+  const R invr = SEP ? rsqrt_sep(r2) : rsqrt_pos(r2);
   if constexpr (KIND == LAPLACE) {
     a[0] += qr * invr;
   } else if constexpr (KIND == YUKAWA) {
@@ -168,8 +168,10 @@ __global__ __launch_bounds__(64) void k_eval_targets(DevTree t, Wavek wk, const 
         R part[NA];
 #pragma unroll
         for (int a = 0; a < NA; a++) part[a] = 0;
+        // equivalent surfaces never touch the targets: no r = 0 test needed
         for (int j = 0; j < m; j++)
-          pair_grad<KIND, R>(sx[j] - tx, sy[j] - ty, sz[j] - tz, sq[j], C ? sqi[j] : R(0), kr, ki, part);
+          pair_grad<KIND, R, MODE != MODE_P2P>(sx[j] - tx, sy[j] - ty, sz[j] - tz, sq[j], C ? sqi[j] : R(0), kr,
+                                               ki, part);
         fold_tile<R, NA>(acc, comp, part);
       }
     }
@@ -226,8 +228,9 @@ __global__ __launch_bounds__(64) void k_eval_surface(DevTree t, Wavek wk, const 
         R part[NA];
 #pragma unroll
         for (int a = 0; a < NA; a++) part[a] = 0;
+        // check surfaces are well separated from the source particles
         for (int j = 0; j < m; j++)
-          pair_pot<KIND, R>(sx[j] - px, sy[j] - py, sz[j] - pz, sq[j], C ? sqi[j] : R(0), kr, ki, part);
+          pair_pot<KIND, R, true>(sx[j] - px, sy[j] - py, sz[j] - pz, sq[j], C ? sqi[j] : R(0), kr, ki, part);
         fold_tile<R, NA>(acc, comp, part);
       }
     }
@@ -317,11 +320,8 @@ static void eval_targets_t(int mode, const DevTree& t, Wavek k, const int* work,
   else hipLaunchKernelGGL((k_eval_targets<KIND, R, MODE_L2P>), dim3(nwork), dim3(64), 0, s, t, k, work);
 }
 
-void launch_eval_targets(int kind, bool mixed, int mode, const DevTree& t, Wavek k, const int* work, int nwork,
+void launch_eval_targets(int kind, bool f32, int mode, const DevTree& t, Wavek k, const int* work, int nwork,
                          hipStream_t s) {
-  // Only the near field (P2P) runs in FP32 in the mixed path; the equivalent
-  // surfaces (M2P, L2P) carry the FP64 pseudo-inverse output and stay FP64.
-  const bool f32 = mixed && mode == MODE_P2P;
   switch (kind) {
     case LAPLACE: f32 ? eval_targets_t<LAPLACE, float>(mode, t, k, work, nwork, s)
                       : eval_targets_t<LAPLACE, double>(mode, t, k, work, nwork, s); break;
@@ -332,21 +332,23 @@ void launch_eval_targets(int kind, bool mixed, int mode, const DevTree& t, Wavek
   }
 }
 
-template <int KIND>
+template <int KIND, typename R>
 static void eval_surface_t(int mode, const DevTree& t, Wavek k, const int* work, int nwork, double* check,
                            hipStream_t s) {
   if (nwork <= 0) return;
-  if (mode == MODE_P2M) hipLaunchKernelGGL((k_eval_surface<KIND, double, MODE_P2M>), dim3(nwork), dim3(64), 0, s, t, k, work, check);
-  else hipLaunchKernelGGL((k_eval_surface<KIND, double, MODE_P2L>), dim3(nwork), dim3(64), 0, s, t, k, work, check);
+  if (mode == MODE_P2M) hipLaunchKernelGGL((k_eval_surface<KIND, R, MODE_P2M>), dim3(nwork), dim3(64), 0, s, t, k, work, check);
+  else hipLaunchKernelGGL((k_eval_surface<KIND, R, MODE_P2L>), dim3(nwork), dim3(64), 0, s, t, k, work, check);
 }
 
-void launch_eval_surface(int kind, bool mixed, int mode, const DevTree& t, Wavek k, const int* work, int nwork,
+void launch_eval_surface(int kind, bool f32, int mode, const DevTree& t, Wavek k, const int* work, int nwork,
                          double* check, hipStream_t s) {
-  (void)mixed;  // check potentials feed the FP64 pseudo-inverse solves: always FP64
   switch (kind) {
-    case LAPLACE: eval_surface_t<LAPLACE>(mode, t, k, work, nwork, check, s); break;
-    case YUKAWA: eval_surface_t<YUKAWA>(mode, t, k, work, nwork, check, s); break;
-    default: eval_surface_t<HELMHOLTZ>(mode, t, k, work, nwork, check, s); break;
+    case LAPLACE: f32 ? eval_surface_t<LAPLACE, float>(mode, t, k, work, nwork, check, s)
+                      : eval_surface_t<LAPLACE, double>(mode, t, k, work, nwork, check, s); break;
+    case YUKAWA: f32 ? eval_surface_t<YUKAWA, float>(mode, t, k, work, nwork, check, s)
+                     : eval_surface_t<YUKAWA, double>(mode, t, k, work, nwork, check, s); break;
+    default: f32 ? eval_surface_t<HELMHOLTZ, float>(mode, t, k, work, nwork, check, s)
+                 : eval_surface_t<HELMHOLTZ, double>(mode, t, k, work, nwork, check, s); break;
   }
 }
 
