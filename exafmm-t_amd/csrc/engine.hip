@@ -69,7 +69,7 @@ struct xfmm_context {
   int kind = 0, p = 0, prec = 0, device = 0;
   // stages that run in FP32 (mixed path only): near-field P2P, M2L
   // (transforms + Hadamard), and the surface evaluations P2M/L2P/M2P/P2L
-  bool f32_p2p = false, f32_m2l = false, f32_surf = false;
+  bool f32_p2p = false, f32_m2l = false, f32_surf = false, f32_m2l_fft = false;
   double wk_re = 0, wk_im = 0;
   int nsurf = 0, nconv = 0, nfreq = 0, n1 = 0, vw = 1;
   bool cplx = false;
@@ -161,6 +161,7 @@ struct xfmm_context {
     m.nsurf = nsurf;
     m.cplx = cplx;
     m.mixed = f32_m2l;
+    m.mixed_fft = f32_m2l_fft;
     m.map_up = map_up.ptr;
     m.map_dn = map_dn.ptr;
     m.fft_in = fft_in.ptr;
@@ -396,13 +397,15 @@ int xfmm_create(int kernel, int p, double wavek_re, double wavek_im, int precisi
   c->p = p;
   c->prec = precision;
   if (precision == XFMM_MIXED) {
-    // default mixed policy: FP32 near field and M2L; XFMM_MIXED_STAGES
-    // (comma list of p2p, m2l, surf) overrides it
+    // default mixed policy: FP32 near field, FP32 M2L spectra and Hadamard
+    // (transform arithmetic FP64); XFMM_MIXED_STAGES (comma list of p2p, m2l,
+    // fft = FP32 transform arithmetic, surf) overrides it
     const char* env = std::getenv("XFMM_MIXED_STAGES");
     const std::string st = env ? env : "p2p,m2l";
     c->f32_p2p = st.find("p2p") != std::string::npos;
     c->f32_m2l = st.find("m2l") != std::string::npos;
     c->f32_surf = st.find("surf") != std::string::npos;
+    c->f32_m2l_fft = c->f32_m2l && st.find("fft") != std::string::npos;
   }
   c->device = device;
   c->wk_re = wavek_re;

@@ -71,7 +71,8 @@ size_t gemm_scratch_doubles(int m, int n, int k);
 struct M2LDev {
   int p = 0, n1 = 0, nfreq = 0, nsurf = 0;
   bool cplx = false;                 // Helmholtz: complex c2c transforms
-  bool mixed = false;                // FP32 transforms / Hadamard
+  bool mixed = false;                // FP32 spectra and Hadamard
+  bool mixed_fft = false;            // FP32 transform arithmetic too (else FP64 arithmetic, FP32 storage)
   const int* map_up = nullptr;       // [nsurf] local index in the [0,p)^3 input cube
   const int* map_dn = nullptr;       // [nsurf] local index in the [p,2p)^3 output cube
   void* fft_in = nullptr;            // [nsrc_slots][nfreq][8] complex

@@ -196,7 +196,7 @@ __global__ __launch_bounds__(256) void k_ifft_dn(M2LDev m, DevTree t, const int*
 // [node][freq][child] layout.  Each 1-D DFT is register blocked: a thread loads
 // its p (or 2p) inputs once and produces all outputs of the line.
 
-template <typename R, int P>
+template <typename RS, typename R, int P>
 __global__ __launch_bounds__(256) void k_fft_up_node(M2LDev m, DevTree t, const int* __restrict__ src_node,
                                                      int nslots) {
   extern __shared__ char smem_raw[];
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(256) void k_fft_up_node(M2LDev m, DevTree t, const 
     g[c * p3 + m.map_up[i]] = R(up[e]);
   }
   __syncthreads();
-  C2<R>* out = reinterpret_cast<C2<R>*>(m.fft_in) + size_t(slot) * m.nfreq * NCHILD;
+  C2<RS>* out = reinterpret_cast<C2<RS>*>(m.fft_in) + size_t(slot) * m.nfreq * NCHILD;
   for (int kx = 0; kx < nh; kx++) {
     // x pass: A[c][z][y] = sum_x g[c][z][y][x] W^(kx x)
     for (int e = threadIdx.x; e < NCHILD * p * p; e += blockDim.x) {
@@ -255,14 +255,14 @@ __global__ __launch_bounds__(256) void k_fft_up_node(M2LDev m, DevTree t, const 
         C2<R> acc{0, 0};
 #pragma unroll
         for (int z = 0; z < p; z++) cfma(acc, b[z], w[(kz * z) % n1]);
-        out[size_t(kx + nh * (ky + n1 * kz)) * NCHILD + c] = acc;
+        out[size_t(kx + nh * (ky + n1 * kz)) * NCHILD + c] = C2<RS>{RS(acc.x), RS(acc.y)};
       }
     }
     __syncthreads();
   }
 }
 
-template <typename R, int P>
+template <typename RS, typename R, int P>
 __global__ __launch_bounds__(256) void k_ifft_dn_node(M2LDev m, DevTree t, const int* __restrict__ trg_node,
                                                       const double* __restrict__ scale, int nslots) {
   extern __shared__ char smem_raw[];
@@ -276,14 +276,17 @@ __global__ __launch_bounds__(256) void k_ifft_dn_node(M2LDev m, DevTree t, const
   make_twiddles(w, n1, +1);
   for (int e = threadIdx.x; e < NCHILD * p3; e += blockDim.x) G[e] = R(0);
   __syncthreads();
-  const C2<R>* in = reinterpret_cast<const C2<R>*>(m.fft_out) + size_t(slot) * m.nfreq * NCHILD;
+  const C2<RS>* in = reinterpret_cast<const C2<RS>*>(m.fft_out) + size_t(slot) * m.nfreq * NCHILD;
   for (int kx = 0; kx < nh; kx++) {
     // inverse z pass, thread (c, ky): z' = p + zz
     for (int e = threadIdx.x; e < NCHILD * n1; e += blockDim.x) {
       const int c = e & 7, ky = e >> 3;
       C2<R> v[n1];
 #pragma unroll
-      for (int kz = 0; kz < n1; kz++) v[kz] = in[size_t(kx + nh * (ky + n1 * kz)) * NCHILD + c];
+      for (int kz = 0; kz < n1; kz++) {
+        const C2<RS> u = in[size_t(kx + nh * (ky + n1 * kz)) * NCHILD + c];
+        v[kz] = C2<R>{R(u.x), R(u.y)};
+      }
       for (int zz = 0; zz < p; zz++) {
         C2<R> acc{0, 0};
 #pragma unroll
@@ -334,7 +337,7 @@ static size_t fft_node_lds(const M2LDev& m, size_t rb) {
 
 // per-node transforms: real kernels, p in [3, 12], LDS within the CU's 160 KiB
 static bool fft_node_ok(const M2LDev& m) {
-  return !m.cplx && m.p >= 3 && m.p <= 12 && fft_node_lds(m, m.mixed ? 4 : 8) <= 160 * 1024;
+  return !m.cplx && m.p >= 3 && m.p <= 12 && fft_node_lds(m, (m.mixed && m.mixed_fft) ? 4 : 8) <= 160 * 1024;
 }
 
 template <typename F>
@@ -482,7 +485,7 @@ static size_t ifft_dn_lds(const M2LDev& m, size_t rb) {
 
 size_t m2l_lds_bytes(const M2LDev& m) {
   const size_t rb = m.mixed ? 4 : 8;
-  if (fft_node_ok(m)) return fft_node_lds(m, rb);
+  if (fft_node_ok(m)) return fft_node_lds(m, (m.mixed && m.mixed_fft) ? 4 : 8);
   const size_t a = fft_up_lds(m, rb), b = ifft_dn_lds(m, rb);
   return a > b ? a : b;
 }
@@ -500,15 +503,18 @@ static void allow_lds(K kernel, size_t bytes) {
 void launch_fft_up(const M2LDev& m, const DevTree& t, const int* src_node, int nslots, hipStream_t s) {
   if (nslots <= 0) return;
   if (fft_node_ok(m)) {
-    const size_t lds = fft_node_lds(m, m.mixed ? 4 : 8);
+    const size_t lds = fft_node_lds(m, (m.mixed && m.mixed_fft) ? 4 : 8);
     node_dispatch(m.p, [&](auto P) {
       constexpr int PP = decltype(P)::value;
-      if (m.mixed) {
-        allow_lds(k_fft_up_node<float, PP>, lds);
-        hipLaunchKernelGGL((k_fft_up_node<float, PP>), dim3(nslots), dim3(256), lds, s, m, t, src_node, nslots);
+      if (m.mixed && m.mixed_fft) {
+        allow_lds(k_fft_up_node<float, float, PP>, lds);
+        hipLaunchKernelGGL((k_fft_up_node<float, float, PP>), dim3(nslots), dim3(256), lds, s, m, t, src_node, nslots);
+      } else if (m.mixed) {
+        allow_lds(k_fft_up_node<float, double, PP>, lds);
+        hipLaunchKernelGGL((k_fft_up_node<float, double, PP>), dim3(nslots), dim3(256), lds, s, m, t, src_node, nslots);
       } else {
-        allow_lds(k_fft_up_node<double, PP>, lds);
-        hipLaunchKernelGGL((k_fft_up_node<double, PP>), dim3(nslots), dim3(256), lds, s, m, t, src_node, nslots);
+        allow_lds(k_fft_up_node<double, double, PP>, lds);
+        hipLaunchKernelGGL((k_fft_up_node<double, double, PP>), dim3(nslots), dim3(256), lds, s, m, t, src_node, nslots);
       }
     });
     return;
@@ -530,17 +536,21 @@ void launch_ifft_dn(const M2LDev& m, const DevTree& t, const int* trg_node, cons
                     hipStream_t s) {
   if (nslots <= 0) return;
   if (fft_node_ok(m)) {
-    const size_t lds = fft_node_lds(m, m.mixed ? 4 : 8);
+    const size_t lds = fft_node_lds(m, (m.mixed && m.mixed_fft) ? 4 : 8);
     node_dispatch(m.p, [&](auto P) {
       constexpr int PP = decltype(P)::value;
-      if (m.mixed) {
-        allow_lds(k_ifft_dn_node<float, PP>, lds);
-        hipLaunchKernelGGL((k_ifft_dn_node<float, PP>), dim3(nslots), dim3(256), lds, s, m, t, trg_node, scale,
+      if (m.mixed && m.mixed_fft) {
+        allow_lds(k_ifft_dn_node<float, float, PP>, lds);
+        hipLaunchKernelGGL((k_ifft_dn_node<float, float, PP>), dim3(nslots), dim3(256), lds, s, m, t, trg_node, scale,
+                           nslots);
+      } else if (m.mixed) {
+        allow_lds(k_ifft_dn_node<float, double, PP>, lds);
+        hipLaunchKernelGGL((k_ifft_dn_node<float, double, PP>), dim3(nslots), dim3(256), lds, s, m, t, trg_node, scale,
                            nslots);
       } else {
-        allow_lds(k_ifft_dn_node<double, PP>, lds);
-        hipLaunchKernelGGL((k_ifft_dn_node<double, PP>), dim3(nslots), dim3(256), lds, s, m, t, trg_node, scale,
-                           nslots);
+        allow_lds(k_ifft_dn_node<double, double, PP>, lds);
+        hipLaunchKernelGGL((k_ifft_dn_node<double, double, PP>), dim3(nslots), dim3(256), lds, s, m, t, trg_node,
+                           scale, nslots);
       }
     });
     return;
