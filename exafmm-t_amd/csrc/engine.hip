@@ -397,11 +397,13 @@ int xfmm_create(int kernel, int p, double wavek_re, double wavek_im, int precisi
   c->p = p;
   c->prec = precision;
   if (precision == XFMM_MIXED) {
-    // default mixed policy: FP32 near field, FP32 M2L spectra and Hadamard
-    // (transform arithmetic FP64); XFMM_MIXED_STAGES (comma list of p2p, m2l,
-    // fft = FP32 transform arithmetic, surf) overrides it
+    // default mixed policy: FP32 near field and FP32 M2L (spectra, Hadamard
+    // and transform arithmetic).  XFMM_MIXED_STAGES overrides it with a comma
+    // list of p2p, m2l (FP32 spectra + Hadamard), fft (FP32 transform
+    // arithmetic, else FP64) and surf (FP32 surface evaluations; loses all
+    // accuracy through the ill-conditioned pseudo-inverses, kept for study)
     const char* env = std::getenv("XFMM_MIXED_STAGES");
-    const std::string st = env ? env : "p2p,m2l";
+    const std::string st = env ? env : "p2p,m2l,fft";
     c->f32_p2p = st.find("p2p") != std::string::npos;
     c->f32_m2l = st.find("m2l") != std::string::npos;
     c->f32_surf = st.find("surf") != std::string::npos;
