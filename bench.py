@@ -1,8 +1,9 @@
 """Benchmark: Laplace FMM evaluation, N=2M uniform, p=10, ncrit=128 on one MI355X.
 
 One step = upward_pass + downward_pass of the whole tree with charges already
-resident in HBM (the FMM evaluation phase; the host tree/list/precompute setup
-is reported separately).  The headline `value` is the mixed-precision path
+resident in HBM (the FMM evaluation phase, every operator including the
+near-field P2P, which each upward_pass starts anew on a second stream; the
+host tree/list/precompute setup is reported separately).  The headline `value` is the mixed-precision path
 (FP32 P2P and M2L, FP64 pseudo-inverse solves and translations); the plain
 FP64 path is timed beside it.  Errors are relative L2 against an FP64 direct
 sum on 2,000 seeded sample targets computed by the engine's direct-sum kernel.

@@ -87,55 +87,77 @@ __global__ __launch_bounds__(256) void k_gemm_rows_cplx(int m, int n, int k, con
   }
 }
 
-// FP64 MFMA path (v_mfma_f64_16x16x4_f64): 64 x 64 output tile per 256-thread
-// workgroup, each wave a 32 x 32 sub-tile (2 x 2 MFMA tiles), K staged through
-// LDS in steps of 16.  Optional split-K (grid.z): each split writes its partial
-// product to `partial` and k_gemm_reduce combines them in a fixed order.
+// FP64 MFMA path (v_mfma_f64_16x16x4_f64): 128 x 64 output tile per
+// 256-thread workgroup, each wave a 64 x 32 sub-tile (4 x 2 MFMA tiles), K
+// staged through LDS in steps of 16 with the next step's global loads held in
+// registers while the current step computes.  Tiles are ordered so that the 8
+// column tiles of one 128-row band run on one XCD (workgroup w goes to XCD
+// w % 8), which then reads that band of A once into its L2.  Optional split-K
+// (grid.y): each split writes its partial product to `partial` and
+// k_gemm_reduce combines them in a fixed order.
 typedef double d4 __attribute__((ext_vector_type(4)));
+constexpr int GM_BM = 128, GM_BN = 64, GM_BK = 16;
 
-__global__ __launch_bounds__(256) void k_gemm_rows_mfma64(int m, int n, int k, int kper, const double* __restrict__ A,
-                                                         const long* __restrict__ arow, const double* __restrict__ B,
-                                                         double* __restrict__ C, const long* __restrict__ crow,
+__global__ __launch_bounds__(256) void k_gemm_rows_mfma64(int m, int n, int k, int kper, int ncol,
+                                                         const double* __restrict__ A, const long* __restrict__ arow,
+                                                         const double* __restrict__ B, double* __restrict__ C,
+                                                         const long* __restrict__ crow,
                                                          const double* __restrict__ alpha, int beta,
                                                          double* __restrict__ partial) {
-  constexpr int BM = 64, BN = 64, BK = 16;
+  constexpr int BM = GM_BM, BN = GM_BN, BK = GM_BK;
   __shared__ double As[BK][BM + 2];
   __shared__ double Bs[BK][BN + 2];
   __shared__ long arow_s[BM];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int w = blockIdx.x, x = w & 7, i = w >> 3;
+  const int row0 = ((i / ncol) * 8 + x) * BM, col0 = (i % ncol) * BN;
+  if (row0 >= m) return;  // padding bands of the XCD ordering (uniform per workgroup)
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int j = lane & 15, g = lane >> 4;
-  const int wr = w >> 1, wc = w & 1;
-  const int row0 = blockIdx.y * BM, col0 = blockIdx.x * BN;
-  const int kbeg = blockIdx.z * kper, kend = min(k, kbeg + kper);
+  const int wr = wv >> 1, wc = wv & 1;
+  const int kbeg = blockIdx.y * kper, kend = min(k, kbeg + kper);
   if (tid < BM) arow_s[tid] = (row0 + tid < m) ? arow[row0 + tid] : -1;
-  d4 acc[2][2];
+  d4 acc[4][2];
 #pragma unroll
-  for (int a = 0; a < 2; a++)
+  for (int a = 0; a < 4; a++)
 #pragma unroll
     for (int b = 0; b < 2; b++) acc[a][b] = d4{0, 0, 0, 0};
   __syncthreads();
-  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+  // loader mapping: A element e = tid + 256 q -> row e / 16, k e % 16;
+  // B element e -> k e / 64, column e % 64
+  const int akk = tid & 15, ar0 = tid >> 4;
+  const int bkk = tid >> 6, bc = tid & 63;
+  long arw[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) arw[q] = arow_s[ar0 + 16 * q];
+  const bool bcol_ok = col0 + bc < n;
+  double ra[8], rb[4];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) ra[q] = (arw[q] >= 0 && k0 + akk < kend) ? A[arw[q] + k0 + akk] : 0.0;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      const int e = tid + 256 * q;
-      const int r = e >> 4, kk = e & 15;
-      const long ar = arow_s[r];
-      As[kk][r] = (ar >= 0 && k0 + kk < kend) ? A[ar + k0 + kk] : 0.0;
-      const int kb = e >> 6, cb = e & 63;
-      Bs[kb][cb] = (k0 + kb < kend && col0 + cb < n) ? B[size_t(k0 + kb) * n + col0 + cb] : 0.0;
+      const int kb = bkk + 4 * q;
+      rb[q] = (bcol_ok && k0 + kb < kend) ? B[size_t(k0 + kb) * n + col0 + bc] : 0.0;
     }
+  };
+  if (kbeg < kend) fetch(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) As[akk][ar0 + 16 * q] = ra[q];
+#pragma unroll
+    for (int q = 0; q < 4; q++) Bs[bkk + 4 * q][bc] = rb[q];
     __syncthreads();
+    if (k0 + BK < kend) fetch(k0 + BK);
 #pragma unroll
     for (int s = 0; s < BK / 4; s++) {
       const int kk = 4 * s + g;
-      double av[2], bv[2];
+      double av[4], bv[2];
 #pragma unroll
-      for (int t = 0; t < 2; t++) {
-        av[t] = As[kk][wr * 32 + t * 16 + j];
-        bv[t] = Bs[kk][wc * 32 + t * 16 + j];
-      }
+      for (int t = 0; t < 4; t++) av[t] = As[kk][wr * 64 + t * 16 + j];
 #pragma unroll
-      for (int a = 0; a < 2; a++)
+      for (int t = 0; t < 2; t++) bv[t] = Bs[kk][wc * 32 + t * 16 + j];
+#pragma unroll
+      for (int a = 0; a < 4; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
     }
@@ -143,20 +165,21 @@ __global__ __launch_bounds__(256) void k_gemm_rows_mfma64(int m, int n, int k, i
   }
   // result register q of lane (g, j): row g + 4q, column j of the 16 x 16 tile
 #pragma unroll
-  for (int a = 0; a < 2; a++)
+  for (int a = 0; a < 4; a++)
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      const int r = row0 + wr * 32 + a * 16 + g + 4 * q;
+      const int r = row0 + wr * 64 + a * 16 + g + 4 * q;
       if (r >= m) continue;
+      const long cr = partial ? 0 : crow[r];
+      const double al = (partial || !alpha) ? 1.0 : alpha[r];
 #pragma unroll
       for (int b = 0; b < 2; b++) {
         const int cc = col0 + wc * 32 + b * 16 + j;
         if (cc >= n) continue;
         if (partial) {
-          partial[(size_t(blockIdx.z) * m + r) * n + cc] = acc[a][b][q];
+          partial[(size_t(blockIdx.y) * m + r) * n + cc] = acc[a][b][q];
         } else {
-          const long cr = crow[r];
-          double v = (alpha ? alpha[r] : 1.0) * acc[a][b][q];
+          double v = al * acc[a][b][q];
           if (beta) v += C[cr + cc];
           C[cr + cc] = v;
         }
@@ -177,10 +200,17 @@ __global__ void k_gemm_reduce(int m, int n, int nsplit, const double* __restrict
   C[cr + cc] = v;
 }
 
-size_t gemm_scratch_doubles(int m, int n, int k) {
-  const int tiles = ((n + 63) / 64) * ((m + 63) / 64);
+// split-K factor: enough workgroups to fill the chip (>= 1024), each split
+// keeping at least 64 of K (4 LDS stages)
+static int gemm_split(int m, int n, int k) {
+  const int tiles = ((n + GM_BN - 1) / GM_BN) * ((m + GM_BM - 1) / GM_BM);
   int split = 1;
-  while (split < 16 && tiles * split < 1024 && k / (split * 2) >= 256) split *= 2;
+  while (split < 64 && tiles * split < 1024 && k / (split * 2) >= 64) split *= 2;
+  return split;
+}
+
+size_t gemm_scratch_doubles(int m, int n, int k) {
+  const int split = gemm_split(m, n, k);
   return split > 1 ? size_t(split) * m * n : 0;
 }
 
@@ -189,15 +219,15 @@ void launch_gemm_rows(bool cplx, int m, int n, int k, const double* A, const lon
                       size_t scratch_doubles) {
   if (m <= 0 || n <= 0) return;
   if (!cplx) {
-    const int tiles = ((n + 63) / 64) * ((m + 63) / 64);
-    int split = 1;
-    while (split < 16 && tiles * split < 1024 && k / (split * 2) >= 256) split *= 2;
+    int split = gemm_split(m, n, k);
     if (split > 1 && (!scratch || size_t(split) * m * n > scratch_doubles)) split = 1;
     int kper = (k + split - 1) / split;
-    kper = (kper + 15) / 16 * 16;
-    dim3 grid((n + 63) / 64, (m + 63) / 64, split);
-    hipLaunchKernelGGL(k_gemm_rows_mfma64, grid, dim3(256), 0, s, m, n, k, kper, A, arow, B, C, crow, alpha, beta,
-                       split > 1 ? scratch : nullptr);
+    kper = (kper + GM_BK - 1) / GM_BK * GM_BK;
+    const int ncol = (n + GM_BN - 1) / GM_BN;
+    const int nband8 = ((m + GM_BM - 1) / GM_BM + 7) / 8 * 8;
+    dim3 grid(unsigned(nband8 * ncol), split);
+    hipLaunchKernelGGL(k_gemm_rows_mfma64, grid, dim3(256), 0, s, m, n, k, kper, ncol, A, arow, B, C, crow, alpha,
+                       beta, split > 1 ? scratch : nullptr);
     if (split > 1) {
       const size_t tot = size_t(m) * n;
       hipLaunchKernelGGL(k_gemm_reduce, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, s, m, n, split, scratch, C,

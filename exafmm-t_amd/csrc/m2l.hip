@@ -396,9 +396,32 @@ struct MfmaT<double> {
 };
 
 template <typename R>
+struct Quad;
+template <>
+struct Quad<float> {
+  typedef float t __attribute__((ext_vector_type(4)));
+};
+template <>
+struct Quad<double> {
+  typedef double t __attribute__((ext_vector_type(4)));
+};
+
+// Per block of <= 64 target slots and a chunk of HAD_KCH frequencies.  For
+// each frequency the block stages the spectra of its source union and the 26
+// operator matrices in LDS, arranged so that every lane fetches its MFMA
+// operands for one relative position with one vector read each: children g
+// and g+4 sit side by side, {Re c_g, Im c_g, Re c_(g+4), Im c_(g+4)}.  The
+// next frequency's spectra and operators are loaded into registers while the
+// current one computes.  Each wave owns 16 targets; per relative position the
+// complex 8x8 product is one real 16x16x16 product (four 16x16x4 MFMAs):
+//   rows 0..7 of the result = Re of target children, rows 8..15 = Im,
+//   A[row][s] = Re/Im of M[s][row & 7], B[s][t] = Re/Im of the source spectrum.
+template <typename R>
 __global__ __launch_bounds__(256) void k_hadamard(M2LDev m, const R* __restrict__ ops, HadBlocks hb, int nchunk) {
   typedef typename MfmaT<R>::v4 v4;
-  __shared__ C2<R> U[HAD_UCAP * NCHILD];
+  typedef typename Quad<R>::t q4;
+  __shared__ q4 Us[HAD_UCAP * 4];  // [union slot][g]
+  __shared__ q4 Ms[NM2L * 8 * 4];  // [r][target child][g]
   __shared__ short loc[NM2L * 64];
   __shared__ unsigned tile_mask[4];
   // XCD-aware (chunk, block) mapping: XCD x = w % 8 owns chunks x, x+8, ...
@@ -407,11 +430,11 @@ __global__ __launch_bounds__(256) void k_hadamard(M2LDev m, const R* __restrict_
   const int chunk = (i / hb.nblk) * 8 + x;
   const int b = i % hb.nblk;
   if (chunk >= nchunk) return;
+  const int kbeg = chunk * HAD_KCH, kend = min(m.nfreq, kbeg + HAD_KCH);
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int j = lane & 15, g = lane >> 4;
   const int nu = hb.nunion[b];
-  const int* uni = hb.uni + size_t(b) * HAD_UCAP;
   for (int e = tid; e < NM2L * 64; e += 256) loc[e] = hb.loc[size_t(b) * NM2L * 64 + e];
   __syncthreads();
   if (tid < 4) {
@@ -421,42 +444,78 @@ __global__ __launch_bounds__(256) void k_hadamard(M2LDev m, const R* __restrict_
         if (loc[r * 64 + tid * 16 + t] >= 0) { msk |= 1u << r; break; }
     tile_mask[tid] = msk;
   }
-  const int trg = hb.trg[size_t(b) * 64 + wave * 16 + j];
   const C2<R>* fin = reinterpret_cast<const C2<R>*>(m.fft_in);
   C2<R>* fout = reinterpret_cast<C2<R>*>(m.fft_out);
   const C2<R>* M = reinterpret_cast<const C2<R>*>(ops);
+  // staging assignment: spectra element e = tid + 256 q (q < 8) is child
+  // e % 8 of union slot e / 8; operator element e (q < 7) is entry e % 64 of
+  // relative position e / 64
+  constexpr int NU_Q = HAD_UCAP * NCHILD / 256, NM_Q = (NM2L * 64 + 255) / 256;
+  const int c8 = tid & 7;
+  int usrc[NU_Q];
+#pragma unroll
+  for (int q = 0; q < NU_Q; q++) {
+    const int e = tid + 256 * q;
+    usrc[q] = (e < nu * NCHILD) ? hb.uni[size_t(b) * HAD_UCAP + (e >> 3)] : -1;
+  }
+  C2<R> ru[NU_Q], rm[NM_Q];
+  auto fetch = [&](int k) {
+#pragma unroll
+    for (int q = 0; q < NU_Q; q++)
+      if (usrc[q] >= 0) ru[q] = fin[(size_t(usrc[q]) * m.nfreq + k) * NCHILD + c8];
+#pragma unroll
+    for (int q = 0; q < NM_Q; q++) {
+      const int e = tid + 256 * q;
+      if (e < NM2L * 64) rm[q] = M[(size_t(e >> 6) * m.nfreq + k) * 64 + (e & 63)];
+    }
+  };
+  R* Usr = reinterpret_cast<R*>(Us);
+  R* Msr = reinterpret_cast<R*>(Ms);
+  const int cpos = 4 * (c8 & 3) + 2 * (c8 >> 2);
   const int j1 = j & 7;
   const bool im_out = j >= 8;
-  for (int it = 0; it < HAD_KCH; it++) {
-    const int k = chunk * HAD_KCH + it;
-    if (k >= m.nfreq) break;
+  const int trg = hb.trg[size_t(b) * 64 + wave * 16 + j];
+  fetch(kbeg);
+  for (int k = kbeg; k < kend; k++) {
     __syncthreads();
-    for (int e = tid; e < nu * NCHILD; e += 256)
-      U[e] = fin[(size_t(uni[e >> 3]) * m.nfreq + k) * NCHILD + (e & 7)];
+#pragma unroll
+    for (int q = 0; q < NU_Q; q++)
+      if (usrc[q] >= 0) {
+        const int e = tid + 256 * q;
+        Usr[(e >> 3) * 16 + cpos] = ru[q].x;
+        Usr[(e >> 3) * 16 + cpos + 1] = ru[q].y;
+      }
+#pragma unroll
+    for (int q = 0; q < NM_Q; q++) {
+      const int e = tid + 256 * q;
+      if (e < NM2L * 64) {
+        // entry (s, t) = operator from source child s to target child t
+        const int r = e >> 6, s = (e >> 3) & 7, t = e & 7;
+        const int o = ((r * 8 + t) * 4 + (s & 3)) * 4 + 2 * (s >> 2);
+        Msr[o] = rm[q].x;
+        Msr[o + 1] = rm[q].y;
+      }
+    }
     __syncthreads();
+    if (k + 1 < kend) fetch(k + 1);
     const unsigned msk = tile_mask[wave];
     if (msk == 0) continue;
     v4 acc = {0, 0, 0, 0};
     for (int r = 0; r < NM2L; r++) {
       if (!((msk >> r) & 1u)) continue;
-      const C2<R>* Mk = M + (size_t(r) * m.nfreq + k) * (NCHILD * NCHILD);
-      const C2<R> m0 = Mk[g * NCHILD + j1];
-      const C2<R> m1 = Mk[(g + 4) * NCHILD + j1];
+      const q4 mm = Ms[(r * 8 + j1) * 4 + g];
       const int u = loc[r * 64 + wave * 16 + j];
-      C2<R> v0{0, 0}, v1{0, 0};
-      if (u >= 0) {
-        v0 = U[u * NCHILD + g];
-        v1 = U[u * NCHILD + g + 4];
-      }
-      acc = MfmaT<R>::mma(im_out ? m0.y : m0.x, v0.x, acc);
-      acc = MfmaT<R>::mma(im_out ? m1.y : m1.x, v1.x, acc);
-      acc = MfmaT<R>::mma(im_out ? m0.x : -m0.y, v0.y, acc);
-      acc = MfmaT<R>::mma(im_out ? m1.x : -m1.y, v1.y, acc);
+      q4 vv = {0, 0, 0, 0};
+      if (u >= 0) vv = Us[u * 4 + g];
+      acc = MfmaT<R>::mma(im_out ? mm.y : mm.x, vv.x, acc);
+      acc = MfmaT<R>::mma(im_out ? mm.w : mm.z, vv.z, acc);
+      acc = MfmaT<R>::mma(im_out ? mm.x : -mm.y, vv.y, acc);
+      acc = MfmaT<R>::mma(im_out ? mm.z : -mm.w, vv.w, acc);
     }
     // result: lane (g, j) register q holds row MfmaT::row(g, q) of target j;
     // rows 0..7 are Re of children 0..7, rows 8..15 Im.
-    R* o = reinterpret_cast<R*>(fout + (size_t(trg < 0 ? 0 : trg) * m.nfreq + k) * NCHILD);
     if (trg >= 0) {
+      R* o = reinterpret_cast<R*>(fout + (size_t(trg) * m.nfreq + k) * NCHILD);
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const int rw = MfmaT<R>::row(g, q);

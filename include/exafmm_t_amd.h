@@ -81,7 +81,14 @@ typedef struct {
 
 /* Create a context (FmmBase ctor: p, kernel, wavek; reference
  * include/laplace.h:15, include/helmholtz.h:16, include/modified_helmholtz.h:17).
- * wavek_im is used by the Helmholtz kernel only.  device = HIP device ordinal. */
+ * wavek_im is used by the Helmholtz kernel only.  device = HIP device ordinal.
+ * precision: XFMM_FP64 computes everything in FP64.  XFMM_MIXED computes the
+ * near field (P2P) and the M2L transforms and Hadamard products in FP32 and
+ * keeps FP64 wherever a pseudo-inverse amplifies rounding: check potentials
+ * (P2M, P2L), the translation GEMMs (P2M/L2P conversions, M2M, L2L) and the
+ * evaluation of local equivalent densities at particles (L2P, M2P).  The
+ * environment variable XFMM_MIXED_STAGES (comma list of p2p, m2l, fft, far,
+ * surf) overrides the FP32 stage set of XFMM_MIXED when a context is created. */
 int xfmm_create(int kernel, int p, double wavek_re, double wavek_im, int precision, int device,
                 xfmm_context** out);
 void xfmm_destroy(xfmm_context* ctx);
@@ -116,8 +123,10 @@ int xfmm_set_charges_device(xfmm_context* ctx, const void* q_device);
 int xfmm_clear_values(xfmm_context* ctx);
 
 /* FmmBase::upward_pass / downward_pass (include/fmm_base.h:162-200).  Each
- * pass starts by zeroing what it produces (up_equiv, resp. dn_equiv and the
- * target values), so repeated evaluation does not need clear_values.
+ * pass computes from zero, so repeated evaluation does not need clear_values:
+ * upward_pass zeroes up_equiv, dn_equiv and the target values (all before
+ * the near field starts); downward_pass zeroes dn_equiv and the target values
+ * again only if anything wrote them since.
  * upward_pass also starts the near-field P2P of the current charges on the
  * second stream; downward_pass adds it to the target values (setting new
  * charges in between discards it and downward_pass recomputes it). */

@@ -179,3 +179,10 @@ def test_linearity_and_repeatability(require_gpu):
     eng.upward_pass()
     eng.downward_pass()
     assert np.array_equal(eng.values(), res[0])
+    # a second downward pass on the same upward state recomputes from zero
+    eng.downward_pass()
+    assert np.array_equal(eng.values(), res[0])
+    # ... also after the caller overwrote the local expansions
+    eng.set_equiv(None, np.ones_like(eng.equiv()[1]))
+    eng.downward_pass()
+    assert np.array_equal(eng.values(), res[0])
