@@ -110,7 +110,6 @@ struct xfmm_context {
   int n_p2p = 0, n_m2p = 0, n_p2l = 0, n_l2p_eval = 0, n_leaf = 0;
   std::vector<int> leaf_level_off;                        // [depth+2] ranges in w_leaf
   DevBuf<long> leaf_slot_rows, leaf_node_rows;            // i*nsurf, node*nsurf
-  DevBuf<double> leaf_alpha;                              // 0.5^level (Laplace) or 1
   std::vector<int> nl_level_off;                          // [depth+2] ranges of the non-leaf list
   DevBuf<long> nl_rows, nl_child_rows;                    // node*nsurf, child0*nsurf
   // M2L
@@ -128,6 +127,9 @@ struct xfmm_context {
   bool ev_ok = false;
   double t_ms[11] = {0};
 
+  // scale of the P2M / L2P conversion of a level-l leaf: 0.5^l for the
+  // scale-invariant Laplace operators (fmm_scale_invariant.h:176-193, 211-219), else 1
+  double leaf_scale(int l) const { return kind == XFMM_LAPLACE ? std::ldexp(1.0, -l) : 1.0; }
   DevTree dev_tree() const {
     DevTree t;
     t.nnodes = nnodes;
@@ -274,9 +276,9 @@ void op_P2M(xfmm_context* c) {
     if (e <= b) continue;
     const int L = c->op_level(l);
     launch_gemm_rows(c->cplx, e - b, c->nsurf, c->nsurf, c->check.ptr, c->leaf_slot_rows.ptr + b,
-                     c->uc2e_uT.ptr + L * ns2, c->tmp.ptr, c->leaf_slot_rows.ptr + b, nullptr, 0, c->stream);
+                     c->uc2e_uT.ptr + L * ns2, c->tmp.ptr, c->leaf_slot_rows.ptr + b, 1.0, 0, c->stream);
     launch_gemm_rows(c->cplx, e - b, c->nsurf, c->nsurf, c->tmp.ptr, c->leaf_slot_rows.ptr + b,
-                     c->uc2e_vT.ptr + L * ns2, c->up.ptr, c->leaf_node_rows.ptr + b, c->leaf_alpha.ptr + b, 0,
+                     c->uc2e_vT.ptr + L * ns2, c->up.ptr, c->leaf_node_rows.ptr + b, c->leaf_scale(l), 0,
                      c->stream);
   }
 }
@@ -289,7 +291,7 @@ void op_M2M(xfmm_context* c) {
     const int L = c->op_level(l);
     // parent row = [child_0 .. child_7] (contiguous, K = 8 nsurf) x M2Mcat
     launch_gemm_rows(c->cplx, e - b, c->nsurf, 8 * c->nsurf, c->up.ptr, c->nl_child_rows.ptr + b,
-                     c->m2mcat.ptr + L * blk, c->up.ptr, c->nl_rows.ptr + b, nullptr, 1, c->stream,
+                     c->m2mcat.ptr + L * blk, c->up.ptr, c->nl_rows.ptr + b, 1.0, 1, c->stream,
                      c->gemm_scratch.ptr, c->gemm_scratch.n);
   }
 }
@@ -301,7 +303,7 @@ void op_L2L(xfmm_context* c) {
     if (e <= b) continue;
     const int L = c->op_level(l);
     launch_gemm_rows(c->cplx, e - b, 8 * c->nsurf, c->nsurf, c->dn.ptr, c->nl_rows.ptr + b, c->l2lcat.ptr + L * blk,
-                     c->dn.ptr, c->nl_child_rows.ptr + b, nullptr, 1, c->stream, c->gemm_scratch.ptr,
+                     c->dn.ptr, c->nl_child_rows.ptr + b, 1.0, 1, c->stream, c->gemm_scratch.ptr,
                      c->gemm_scratch.n);
   }
 }
@@ -313,9 +315,9 @@ void op_L2P(xfmm_context* c) {
     if (e <= b) continue;
     const int L = c->op_level(l);
     launch_gemm_rows(c->cplx, e - b, c->nsurf, c->nsurf, c->dn.ptr, c->leaf_node_rows.ptr + b,
-                     c->dc2e_uT.ptr + L * ns2, c->tmp.ptr, c->leaf_slot_rows.ptr + b, nullptr, 0, c->stream);
+                     c->dc2e_uT.ptr + L * ns2, c->tmp.ptr, c->leaf_slot_rows.ptr + b, 1.0, 0, c->stream);
     launch_gemm_rows(c->cplx, e - b, c->nsurf, c->nsurf, c->tmp.ptr, c->leaf_slot_rows.ptr + b,
-                     c->dc2e_vT.ptr + L * ns2, c->dn.ptr, c->leaf_node_rows.ptr + b, c->leaf_alpha.ptr + b, 0,
+                     c->dc2e_vT.ptr + L * ns2, c->dn.ptr, c->leaf_node_rows.ptr + b, c->leaf_scale(l), 0,
                      c->stream);
   }
   DevTree t = c->dev_tree();
@@ -341,6 +343,23 @@ void op_P2P(xfmm_context* c) {
 
 void record(xfmm_context* c, int i) {
   if (c->ev_ok) HIPCHK(hipEventRecord(c->ev[i], c->stream));
+}
+
+// Queue the near-field P2P of the current charges on the second stream, after
+// everything already queued on the main stream.
+void launch_near(xfmm_context* c) {
+  if (!c->ev_ok || c->near_pending) return;
+  HIPCHK(hipEventRecord(c->ev[23], c->stream));
+  HIPCHK(hipStreamWaitEvent(c->stream2, c->ev[23], 0));
+  launch_zero(c->near.ptr, c->near.n, c->stream2);
+  HIPCHK(hipEventRecord(c->ev[21], c->stream2));
+  DevTree t = c->dev_tree();
+  t.trg_value = c->near.ptr;
+  launch_eval_targets(c->kind, c->f32_p2p, MODE_P2P, t, c->wavek(), c->w_p2p.ptr, c->n_p2p, c->stream2);
+  HIPCHK(hipEventRecord(c->ev[22], c->stream2));
+  HIPCHK(hipEventRecord(c->ev[20], c->stream2));
+  c->near_pending = true;
+  c->near_launched = true;
 }
 
 void op_M2L(xfmm_context* c, bool timed) {
@@ -649,11 +668,9 @@ int xfmm_set_tree(xfmm_context* c, const xfmm_tree* t) {
     }
     c->leaf_level_off[D + 1] = int(leaf.size());
     std::vector<long> lsr(leaf.size()), lnr(leaf.size());
-    std::vector<double> la(leaf.size());
     for (size_t i = 0; i < leaf.size(); i++) {
       lsr[i] = long(i) * c->nsurf;
       lnr[i] = long(leaf[i]) * c->nsurf;
-      la[i] = (c->kind == XFMM_LAPLACE) ? std::ldexp(1.0, -lev[leaf[i]]) : 1.0;
     }
     c->w_p2p.upload(wp2p, st);
     c->w_m2p.upload(wm2p, st);
@@ -662,7 +679,6 @@ int xfmm_set_tree(xfmm_context* c, const xfmm_tree* t) {
     c->w_leaf.upload(leaf, st);
     c->leaf_slot_rows.upload(lsr, st);
     c->leaf_node_rows.upload(lnr, st);
-    c->leaf_alpha.upload(la, st);
     c->n_p2p = int(wp2p.size());
     c->n_m2p = int(wm2p.size());
     c->n_p2l = int(wp2l.size());
@@ -843,32 +859,21 @@ int xfmm_clear_values(xfmm_context* c) {
 int xfmm_upward_pass(xfmm_context* c) {
   return guarded(c, [&] {
     check_ready(c);
-    // clear the accumulators of both passes before the near field starts on
-    // the second stream, so that these stores do not compete with it
+    // clear the accumulators of both passes (the near field of the previous
+    // evaluation has been consumed by then)
     launch_zero(c->up.ptr, c->up.n, c->stream);
     launch_zero(c->dn.ptr, c->dn.n, c->stream);
     launch_zero(c->trg_value.ptr, c->trg_value.n, c->stream);
     c->down_clean = true;
-    if (c->ev_ok) {
-      // near-field P2P on the second stream, after everything already queued
-      // on the main stream (charges, and the axpy of a previous near field)
-      HIPCHK(hipEventRecord(c->ev[23], c->stream));
-      HIPCHK(hipStreamWaitEvent(c->stream2, c->ev[23], 0));
-      launch_zero(c->near.ptr, c->near.n, c->stream2);
-      HIPCHK(hipEventRecord(c->ev[21], c->stream2));
-      DevTree t = c->dev_tree();
-      t.trg_value = c->near.ptr;
-      launch_eval_targets(c->kind, c->f32_p2p, MODE_P2P, t, c->wavek(), c->w_p2p.ptr, c->n_p2p, c->stream2);
-      HIPCHK(hipEventRecord(c->ev[22], c->stream2));
-      HIPCHK(hipEventRecord(c->ev[20], c->stream2));
-      c->near_pending = true;
-      c->near_launched = true;
-    }
     record(c, 0);
     op_P2M(c);
     record(c, 1);
     op_M2M(c);
     record(c, 2);
+    // the near field goes on the second stream here, so that it overlaps
+    // the downward pass (measured: starting it with the upward pass, where it
+    // competes with the VALU-bound P2M check potentials, is slower)
+    launch_near(c);
     HIPCHK(hipGetLastError());
   });
 }

@@ -23,7 +23,7 @@ template <int BM, int BN, int TM, int TN, int BK>
 __global__ __launch_bounds__(256) void k_gemm_rows_cplx(int m, int n, int k, const double2* __restrict__ A,
                                                        const long* __restrict__ arow, const double2* __restrict__ B,
                                                        double2* __restrict__ C, const long* __restrict__ crow,
-                                                       const double* __restrict__ alpha, int beta) {
+                                                       double alpha, int beta) {
   static_assert((BM / TM) * (BN / TN) == 256, "thread layout");
   __shared__ double2 As[BK][BM + 1];
   __shared__ double2 Bs[BK][BN];
@@ -75,7 +75,7 @@ __global__ __launch_bounds__(256) void k_gemm_rows_cplx(int m, int n, int k, con
     const int r = row0 + tr * TM + i;
     if (r >= m) continue;
     const long cr = crow[r];
-    const double al = alpha ? alpha[r] : 1.0;
+    const double al = alpha;
 #pragma unroll
     for (int j = 0; j < TN; j++) {
       const int c = col0 + tc * TN + j;
@@ -87,123 +87,146 @@ __global__ __launch_bounds__(256) void k_gemm_rows_cplx(int m, int n, int k, con
   }
 }
 
-// FP64 MFMA path (v_mfma_f64_16x16x4_f64): 128 x 64 output tile per
-// 256-thread workgroup, each wave a 64 x 32 sub-tile (4 x 2 MFMA tiles), K
-// staged through LDS in steps of 16 with the next step's global loads held in
-// registers while the current step computes.  Tiles are ordered so that the 8
-// column tiles of one 128-row band run on one XCD (workgroup w goes to XCD
-// w % 8), which then reads that band of A once into its L2.  Optional split-K
-// (grid.y): each split writes its partial product to `partial` and
-// k_gemm_reduce combines them in a fixed order.
+// FP64 MFMA path (v_mfma_f64_16x16x4_f64): BM x BN output tile per 256-thread
+// workgroup, the 4 waves as 2 x 2 sub-tiles of (BM/2) x (BN/2) (a grid of
+// 16 x 16 MFMA tiles each), K staged through LDS in steps of 16 with the next
+// step's global loads held in registers while the current step computes.
+// Tiles are ordered so that the column tiles of one BM-row band run on one
+// XCD (workgroup w goes to XCD w % 8), which then reads that band of A once
+// into its L2.  Optional split-K (grid.y): each split writes its partial
+// product to `partial` and k_gemm_reduce combines them in a fixed order.
 typedef double d4 __attribute__((ext_vector_type(4)));
-constexpr int GM_BM = 128, GM_BN = 64, GM_BK = 16;
+constexpr int GM_BK = 16;
 
+// Without `partial` the kernel writes C itself: alpha scales the rows of A as
+// they are staged (exact: alpha is a power of two) and BETA adds C in the
+// epilogue.  BETA = false keeps the epilogue a plain store, so the
+// accumulators never leave the AGPRs before it (BETA = true costs occupancy
+// and is used only when no split-K scratch is available).
+template <int BM, int BN, bool BETA>
 __global__ __launch_bounds__(256) void k_gemm_rows_mfma64(int m, int n, int k, int kper, int ncol,
                                                          const double* __restrict__ A, const long* __restrict__ arow,
                                                          const double* __restrict__ B, double* __restrict__ C,
-                                                         const long* __restrict__ crow,
-                                                         const double* __restrict__ alpha, int beta,
+                                                         const long* __restrict__ crow, double alpha,
                                                          double* __restrict__ partial) {
-  constexpr int BM = GM_BM, BN = GM_BN, BK = GM_BK;
+  constexpr int BK = GM_BK, WM = BM / 2, WN = BN / 2, TA = WM / 16, TB = WN / 16;
+  constexpr int NQA = BM * BK / 256, NQB = BK * BN / 256;  // loader elements per thread
+  constexpr int ARS = 256 / BK, BKS = 256 / BN;            // loader row / k strides
+  static_assert(BM % 32 == 0 && BN % 32 == 0 && BN <= 256 && NQA >= 1 && NQB >= 1, "tile shape");
   __shared__ double As[BK][BM + 2];
   __shared__ double Bs[BK][BN + 2];
   __shared__ long arow_s[BM];
-  const int w = blockIdx.x, x = w & 7, i = w >> 3;
-  const int row0 = ((i / ncol) * 8 + x) * BM, col0 = (i % ncol) * BN;
+  // XCD ordering when there are at least 8 row bands (grid padded to a
+  // multiple of 8 bands), plain row-major tile order otherwise
+  const int w = blockIdx.x;
+  const int nband = (m + BM - 1) / BM;
+  int row0, col0;
+  if (nband >= 8) {
+    const int x = w & 7, i = w >> 3;
+    row0 = ((i / ncol) * 8 + x) * BM;
+    col0 = (i % ncol) * BN;
+  } else {
+    row0 = (w / ncol) * BM;
+    col0 = (w % ncol) * BN;
+  }
   if (row0 >= m) return;  // padding bands of the XCD ordering (uniform per workgroup)
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int j = lane & 15, g = lane >> 4;
   const int wr = wv >> 1, wc = wv & 1;
   const int kbeg = blockIdx.y * kper, kend = min(k, kbeg + kper);
-  if (tid < BM) arow_s[tid] = (row0 + tid < m) ? arow[row0 + tid] : -1;
-  d4 acc[4][2];
+  for (int r = tid; r < BM; r += 256) arow_s[r] = (row0 + r < m) ? arow[row0 + r] : -1;
+  const bool direct = partial == nullptr;
+  const double scale = direct ? alpha : 1.0;
+  d4 acc[TA][TB];
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int a = 0; a < TA; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++) acc[a][b] = d4{0, 0, 0, 0};
+    for (int b = 0; b < TB; b++) acc[a][b] = d4{0, 0, 0, 0};
   __syncthreads();
-  // loader mapping: A element e = tid + 256 q -> row e / 16, k e % 16;
-  // B element e -> k e / 64, column e % 64
-  const int akk = tid & 15, ar0 = tid >> 4;
-  const int bkk = tid >> 6, bc = tid & 63;
-  long arw[8];
+  // loader mapping: A element e = tid + 256 q -> row e / BK, k e % BK;
+  // B element e -> k e / BN, column e % BN
+  const int akk = tid % BK, ar0 = tid / BK;
+  const int bkk = tid / BN, bc = tid % BN;
+  long arw[NQA];
 #pragma unroll
-  for (int q = 0; q < 8; q++) arw[q] = arow_s[ar0 + 16 * q];
+  for (int q = 0; q < NQA; q++) arw[q] = arow_s[ar0 + ARS * q];
   const bool bcol_ok = col0 + bc < n;
-  double ra[8], rb[4];
+  double ra[NQA], rb[NQB];
   auto fetch = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < 8; q++) ra[q] = (arw[q] >= 0 && k0 + akk < kend) ? A[arw[q] + k0 + akk] : 0.0;
+    for (int q = 0; q < NQA; q++) ra[q] = (arw[q] >= 0 && k0 + akk < kend) ? scale * A[arw[q] + k0 + akk] : 0.0;
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int kb = bkk + 4 * q;
+    for (int q = 0; q < NQB; q++) {
+      const int kb = bkk + BKS * q;
       rb[q] = (bcol_ok && k0 + kb < kend) ? B[size_t(k0 + kb) * n + col0 + bc] : 0.0;
     }
   };
   if (kbeg < kend) fetch(kbeg);
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
 #pragma unroll
-    for (int q = 0; q < 8; q++) As[akk][ar0 + 16 * q] = ra[q];
+    for (int q = 0; q < NQA; q++) As[akk][ar0 + ARS * q] = ra[q];
 #pragma unroll
-    for (int q = 0; q < 4; q++) Bs[bkk + 4 * q][bc] = rb[q];
+    for (int q = 0; q < NQB; q++) Bs[bkk + BKS * q][bc] = rb[q];
     __syncthreads();
     if (k0 + BK < kend) fetch(k0 + BK);
 #pragma unroll
     for (int s = 0; s < BK / 4; s++) {
       const int kk = 4 * s + g;
-      double av[4], bv[2];
+      double av[TA], bv[TB];
 #pragma unroll
-      for (int t = 0; t < 4; t++) av[t] = As[kk][wr * 64 + t * 16 + j];
+      for (int t = 0; t < TA; t++) av[t] = As[kk][wr * WM + t * 16 + j];
 #pragma unroll
-      for (int t = 0; t < 2; t++) bv[t] = Bs[kk][wc * 32 + t * 16 + j];
+      for (int t = 0; t < TB; t++) bv[t] = Bs[kk][wc * WN + t * 16 + j];
 #pragma unroll
-      for (int a = 0; a < 4; a++)
+      for (int a = 0; a < TA; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+        for (int b = 0; b < TB; b++) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
     }
     __syncthreads();
   }
   // result register q of lane (g, j): row g + 4q, column j of the 16 x 16 tile
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int a = 0; a < TA; a++)
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      const int r = row0 + wr * 64 + a * 16 + g + 4 * q;
+      const int r = row0 + wr * WM + a * 16 + g + 4 * q;
       if (r >= m) continue;
-      const long cr = partial ? 0 : crow[r];
-      const double al = (partial || !alpha) ? 1.0 : alpha[r];
 #pragma unroll
-      for (int b = 0; b < 2; b++) {
-        const int cc = col0 + wc * 32 + b * 16 + j;
+      for (int b = 0; b < TB; b++) {
+        const int cc = col0 + wc * WN + b * 16 + j;
         if (cc >= n) continue;
-        if (partial) {
-          partial[(size_t(blockIdx.y) * m + r) * n + cc] = acc[a][b][q];
-        } else {
-          double v = al * acc[a][b][q];
-          if (beta) v += C[cr + cc];
-          C[cr + cc] = v;
-        }
+        if (!direct) partial[(size_t(blockIdx.y) * m + r) * n + cc] = acc[a][b][q];
+        else if (BETA) C[crow[r] + cc] += acc[a][b][q];
+        else C[crow[r] + cc] = acc[a][b][q];
       }
     }
 }
 
 __global__ void k_gemm_reduce(int m, int n, int nsplit, const double* __restrict__ partial, double* __restrict__ C,
-                              const long* __restrict__ crow, const double* __restrict__ alpha, int beta) {
+                              const long* __restrict__ crow, double alpha, int beta) {
   const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= size_t(m) * n) return;
   const int r = int(i / n), cc = int(i % n);
   double s = 0;
   for (int z = 0; z < nsplit; z++) s += partial[(size_t(z) * m + r) * n + cc];
   const long cr = crow[r];
-  double v = (alpha ? alpha[r] : 1.0) * s;
+  double v = alpha * s;
   if (beta) v += C[cr + cc];
   C[cr + cc] = v;
 }
 
+// Tile shape: 128 x 128 for large outputs, 64 x 64 for small levels (more
+// workgroups, shorter per-workgroup K loop).
+struct GemmTile {
+  int bm, bn;
+};
+static GemmTile gemm_tile(int m) { return m >= 2048 ? GemmTile{128, 128} : GemmTile{64, 64}; }
+
 // split-K factor: enough workgroups to fill the chip (>= 1024), each split
 // keeping at least 64 of K (4 LDS stages)
 static int gemm_split(int m, int n, int k) {
-  const int tiles = ((n + GM_BN - 1) / GM_BN) * ((m + GM_BM - 1) / GM_BM);
+  const GemmTile t = gemm_tile(m);
+  const int tiles = ((n + t.bn - 1) / t.bn) * ((m + t.bm - 1) / t.bm);
   int split = 1;
   while (split < 64 && tiles * split < 1024 && k / (split * 2) >= 64) split *= 2;
   return split;
@@ -214,21 +237,42 @@ size_t gemm_scratch_doubles(int m, int n, int k) {
   return split > 1 ? size_t(split) * m * n : 0;
 }
 
+template <int BM, int BN, bool BETA>
+static void launch_mfma(int m, int n, int k, int split, const double* A, const long* arow, const double* B, double* C,
+                        const long* crow, double alpha, double* partial, hipStream_t s) {
+  int kper = (k + split - 1) / split;
+  kper = (kper + GM_BK - 1) / GM_BK * GM_BK;
+  const int ncol = (n + BN - 1) / BN;
+  const int nband = (m + BM - 1) / BM;
+  const int nb = nband >= 8 ? (nband + 7) / 8 * 8 : nband;
+  dim3 grid(unsigned(nb * ncol), split);
+  hipLaunchKernelGGL((k_gemm_rows_mfma64<BM, BN, BETA>), grid, dim3(256), 0, s, m, n, k, kper, ncol, A, arow, B, C,
+                     crow, alpha, partial);
+}
+
+template <int BM, int BN>
+static void launch_tile(int m, int n, int k, int split, bool part, const double* A, const long* arow, const double* B,
+                        double* C, const long* crow, double alpha, int beta, double* scratch, hipStream_t s) {
+  if (part) launch_mfma<BM, BN, false>(m, n, k, split, A, arow, B, C, crow, alpha, scratch, s);
+  else if (beta) launch_mfma<BM, BN, true>(m, n, k, 1, A, arow, B, C, crow, alpha, nullptr, s);
+  else launch_mfma<BM, BN, false>(m, n, k, 1, A, arow, B, C, crow, alpha, nullptr, s);
+}
+
 void launch_gemm_rows(bool cplx, int m, int n, int k, const double* A, const long* arow, const double* B,
-                      double* C, const long* crow, const double* alpha, int beta, hipStream_t s, double* scratch,
+                      double* C, const long* crow, double alpha, int beta, hipStream_t s, double* scratch,
                       size_t scratch_doubles) {
   if (m <= 0 || n <= 0) return;
   if (!cplx) {
+    // accumulating products (beta) also go through the partial buffer when
+    // there is one: the reduction then adds C
     int split = gemm_split(m, n, k);
-    if (split > 1 && (!scratch || size_t(split) * m * n > scratch_doubles)) split = 1;
-    int kper = (k + split - 1) / split;
-    kper = (kper + GM_BK - 1) / GM_BK * GM_BK;
-    const int ncol = (n + GM_BN - 1) / GM_BN;
-    const int nband8 = ((m + GM_BM - 1) / GM_BM + 7) / 8 * 8;
-    dim3 grid(unsigned(nband8 * ncol), split);
-    hipLaunchKernelGGL(k_gemm_rows_mfma64, grid, dim3(256), 0, s, m, n, k, kper, ncol, A, arow, B, C, crow, alpha,
-                       beta, split > 1 ? scratch : nullptr);
-    if (split > 1) {
+    const bool part = scratch && (split > 1 || beta) && size_t(split) * m * n <= scratch_doubles;
+    if (!part) split = 1;
+    if (gemm_tile(m).bm == 128)
+      launch_tile<128, 128>(m, n, k, split, part, A, arow, B, C, crow, alpha, beta, scratch, s);
+    else
+      launch_tile<64, 64>(m, n, k, split, part, A, arow, B, C, crow, alpha, beta, scratch, s);
+    if (part) {
       const size_t tot = size_t(m) * n;
       hipLaunchKernelGGL(k_gemm_reduce, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, s, m, n, split, scratch, C,
                          crow, alpha, beta);

@@ -416,14 +416,15 @@ struct Quad<double> {
 // complex 8x8 product is one real 16x16x16 product (four 16x16x4 MFMAs):
 //   rows 0..7 of the result = Re of target children, rows 8..15 = Im,
 //   A[row][s] = Re/Im of M[s][row & 7], B[s][t] = Re/Im of the source spectrum.
+// A lane's union index for each relative position is fixed for the block and
+// held in registers; an absent source points at a zero row of the staging
+// array, so the 26 relative positions run without branches.
 template <typename R>
 __global__ __launch_bounds__(256) void k_hadamard(M2LDev m, const R* __restrict__ ops, HadBlocks hb, int nchunk) {
   typedef typename MfmaT<R>::v4 v4;
   typedef typename Quad<R>::t q4;
-  __shared__ q4 Us[HAD_UCAP * 4];  // [union slot][g]
-  __shared__ q4 Ms[NM2L * 8 * 4];  // [r][target child][g]
-  __shared__ short loc[NM2L * 64];
-  __shared__ unsigned tile_mask[4];
+  __shared__ q4 Us[(HAD_UCAP + 1) * 4];  // [union slot][g]; slot HAD_UCAP stays zero
+  __shared__ q4 Ms[NM2L * 8 * 4];        // [r][target child][g]
   // XCD-aware (chunk, block) mapping: XCD x = w % 8 owns chunks x, x+8, ...
   const int w = blockIdx.x;
   const int x = w & 7, i = w >> 3;
@@ -432,17 +433,15 @@ __global__ __launch_bounds__(256) void k_hadamard(M2LDev m, const R* __restrict_
   if (chunk >= nchunk) return;
   const int kbeg = chunk * HAD_KCH, kend = min(m.nfreq, kbeg + HAD_KCH);
   const int tid = threadIdx.x, lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wave = tid >> 6;
   const int j = lane & 15, g = lane >> 4;
   const int nu = hb.nunion[b];
-  for (int e = tid; e < NM2L * 64; e += 256) loc[e] = hb.loc[size_t(b) * NM2L * 64 + e];
-  __syncthreads();
-  if (tid < 4) {
-    unsigned msk = 0;
-    for (int r = 0; r < NM2L; r++)
-      for (int t = 0; t < 16; t++)
-        if (loc[r * 64 + tid * 16 + t] >= 0) { msk |= 1u << r; break; }
-    tile_mask[tid] = msk;
+  if (tid < 4) Us[HAD_UCAP * 4 + tid] = q4{0, 0, 0, 0};
+  int uoff[NM2L];
+#pragma unroll
+  for (int r = 0; r < NM2L; r++) {
+    const int u = hb.loc[(size_t(b) * NM2L + r) * 64 + wave * 16 + j];
+    uoff[r] = (u >= 0 ? u : HAD_UCAP) * 4 + g;
   }
   const C2<R>* fin = reinterpret_cast<const C2<R>*>(m.fft_in);
   C2<R>* fout = reinterpret_cast<C2<R>*>(m.fft_out);
@@ -498,20 +497,27 @@ __global__ __launch_bounds__(256) void k_hadamard(M2LDev m, const R* __restrict_
     }
     __syncthreads();
     if (k + 1 < kend) fetch(k + 1);
-    const unsigned msk = tile_mask[wave];
-    if (msk == 0) continue;
-    v4 acc = {0, 0, 0, 0};
+    // two accumulators (even / odd r) and operands of r + 1 read while r
+    // computes
+    v4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+    q4 mm = Ms[j1 * 4 + g];
+    q4 vv = Us[uoff[0]];
+#pragma unroll
     for (int r = 0; r < NM2L; r++) {
-      if (!((msk >> r) & 1u)) continue;
-      const q4 mm = Ms[(r * 8 + j1) * 4 + g];
-      const int u = loc[r * 64 + wave * 16 + j];
-      q4 vv = {0, 0, 0, 0};
-      if (u >= 0) vv = Us[u * 4 + g];
+      q4 mn = mm, vn = vv;
+      if (r + 1 < NM2L) {
+        mn = Ms[((r + 1) * 8 + j1) * 4 + g];
+        vn = Us[uoff[r + 1]];
+      }
+      v4& acc = (r & 1) ? acc1 : acc0;
       acc = MfmaT<R>::mma(im_out ? mm.y : mm.x, vv.x, acc);
       acc = MfmaT<R>::mma(im_out ? mm.w : mm.z, vv.z, acc);
       acc = MfmaT<R>::mma(im_out ? mm.x : -mm.y, vv.y, acc);
       acc = MfmaT<R>::mma(im_out ? mm.z : -mm.w, vv.w, acc);
+      mm = mn;
+      vv = vn;
     }
+    const v4 acc = acc0 + acc1;
     // result: lane (g, j) register q holds row MfmaT::row(g, q) of target j;
     // rows 0..7 are Re of children 0..7, rows 8..15 Im.
     if (trg >= 0) {

@@ -23,11 +23,25 @@ namespace xfmm {
 template <int KIND>
 constexpr bool kcplx() { return KIND == HELMHOLTZ; }
 
+// 1/r of a pair.  RSQ_LIB: library rsqrt, r = 0 gives 0 (the direct sum used as
+// accuracy check).  RSQ_MASK: hardware estimate refined to full precision
+// (rsqrt_sep), r = 0 gives 0 (a leaf with itself, where a target may coincide
+// with a source).  RSQ_SEP: rsqrt_sep without the test (pairs that cannot
+// coincide: distinct leaves own distinct points, and surfaces never touch
+// particles).
+enum RsqMode { RSQ_LIB = 0, RSQ_MASK = 1, RSQ_SEP = 2 };
+template <int RM, typename R>
+__device__ __forceinline__ R inv_r(R r2) {
+  if constexpr (RM == RSQ_LIB) return rsqrt_pos(r2);
+  else if constexpr (RM == RSQ_MASK) return r2 > R(0) ? rsqrt_sep(r2) : R(0);
+  else return rsqrt_sep(r2);
+}
+
 // potential + gradient of one pair, accumulated into a[] (4 or 8 values)
-template <int KIND, typename R, bool SEP = false>
+template <int KIND, typename R, int RM>
 __device__ __forceinline__ void pair_grad(R dx, R dy, R dz, R qr, R qi, R kr, R ki, R* a) {
   const R r2 = dx * dx + dy * dy + dz * dz;
-  const R invr = SEP ? rsqrt_sep(r2) : rsqrt_pos(r2);
+  const R invr = inv_r<RM>(r2);
   if constexpr (KIND == LAPLACE) {
     const R qinv = qr * invr;
     a[0] += qinv;
@@ -67,10 +81,10 @@ __device__ __forceinline__ void pair_grad(R dx, R dy, R dz, R qr, R qi, R kr, R 
 }
 
 // potential of one pair into a[0] (and a[1] for Helmholtz)
-template <int KIND, typename R, bool SEP = false>
+template <int KIND, typename R, int RM>
 __device__ __forceinline__ void pair_pot(R dx, R dy, R dz, R qr, R qi, R kr, R ki, R* a) {
   const R r2 = dx * dx + dy * dy + dz * dz;
-  const R invr = SEP ? rsqrt_sep(r2) : rsqrt_pos(r2);
+  const R invr = inv_r<RM>(r2);
   if constexpr (KIND == LAPLACE) {
     a[0] += qr * invr;
   } else if constexpr (KIND == YUKAWA) {
@@ -98,6 +112,32 @@ __device__ __forceinline__ void fold_tile(R* acc, R* comp, const R* part) {
 }
 
 constexpr int TILE = 64;
+
+// Sum the m (<= TILE) staged sources of a tile into part[], in source order,
+// four pairs per loop trip so that the LDS reads of a trip are issued together.
+template <int KIND, typename R, int RM, bool GRAD>
+__device__ __forceinline__ void tile_sum(int m, const R* sx, const R* sy, const R* sz, const R* sq, const R* sqi, R tx,
+                                         R ty, R tz, R kr, R ki, R* part) {
+  constexpr bool C = kcplx<KIND>();
+  int j = 0;
+  for (; j + 4 <= m; j += 4) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if constexpr (GRAD)
+        pair_grad<KIND, R, RM>(sx[j + u] - tx, sy[j + u] - ty, sz[j + u] - tz, sq[j + u], C ? sqi[j + u] : R(0), kr, ki,
+                               part);
+      else
+        pair_pot<KIND, R, RM>(sx[j + u] - tx, sy[j + u] - ty, sz[j + u] - tz, sq[j + u], C ? sqi[j + u] : R(0), kr, ki,
+                              part);
+    }
+  }
+  for (; j < m; j++) {
+    if constexpr (GRAD)
+      pair_grad<KIND, R, RM>(sx[j] - tx, sy[j] - ty, sz[j] - tz, sq[j], C ? sqi[j] : R(0), kr, ki, part);
+    else
+      pair_pot<KIND, R, RM>(sx[j] - tx, sy[j] - ty, sz[j] - tz, sq[j], C ? sqi[j] : R(0), kr, ki, part);
+  }
+}
 
 template <int KIND, typename R, int MODE>
 __global__ __launch_bounds__(64) void k_eval_targets(DevTree t, Wavek wk, const int* __restrict__ work) {
@@ -168,10 +208,12 @@ __global__ __launch_bounds__(64) void k_eval_targets(DevTree t, Wavek wk, const 
         R part[NA];
 #pragma unroll
         for (int a = 0; a < NA; a++) part[a] = 0;
-        // equivalent surfaces never touch the targets: no r = 0 test needed
-        for (int j = 0; j < m; j++)
-          pair_grad<KIND, R, MODE != MODE_P2P>(sx[j] - tx, sy[j] - ty, sz[j] - tz, sq[j], C ? sqi[j] : R(0), kr,
-                                               ki, part);
+        // only a leaf with itself can pair coinciding points; equivalent
+        // surfaces never touch the targets
+        if (MODE == MODE_P2P && snode == leaf)
+          tile_sum<KIND, R, RSQ_MASK, true>(m, sx, sy, sz, sq, sqi, tx, ty, tz, kr, ki, part);
+        else
+          tile_sum<KIND, R, RSQ_SEP, true>(m, sx, sy, sz, sq, sqi, tx, ty, tz, kr, ki, part);
         fold_tile<R, NA>(acc, comp, part);
       }
     }
@@ -229,8 +271,7 @@ __global__ __launch_bounds__(64) void k_eval_surface(DevTree t, Wavek wk, const 
 #pragma unroll
         for (int a = 0; a < NA; a++) part[a] = 0;
         // check surfaces are well separated from the source particles
-        for (int j = 0; j < m; j++)
-          pair_pot<KIND, R, true>(sx[j] - px, sy[j] - py, sz[j] - pz, sq[j], C ? sqi[j] : R(0), kr, ki, part);
+        tile_sum<KIND, R, RSQ_SEP, false>(m, sx, sy, sz, sq, sqi, px, py, pz, kr, ki, part);
         fold_tile<R, NA>(acc, comp, part);
       }
     }
@@ -288,8 +329,8 @@ __global__ __launch_bounds__(64) void k_direct(const double* __restrict__ src, c
     for (int a = 0; a < NA; a++) part[a] = 0;
     for (int j = 0; j < m; j++) {
       const R dx = R(sx[j] - tx), dy = R(sy[j] - ty), dz = R(sz[j] - tz);
-      if (GRAD) pair_grad<KIND, R>(dx, dy, dz, sq[j], C ? sqi[j] : R(0), kr, ki, part);
-      else pair_pot<KIND, R>(dx, dy, dz, sq[j], C ? sqi[j] : R(0), kr, ki, part);
+      if (GRAD) pair_grad<KIND, R, RSQ_LIB>(dx, dy, dz, sq[j], C ? sqi[j] : R(0), kr, ki, part);
+      else pair_pot<KIND, R, RSQ_LIB>(dx, dy, dz, sq[j], C ? sqi[j] : R(0), kr, ki, part);
     }
 #pragma unroll
     for (int a = 0; a < NA; a++) kahan_add(acc[a], comp[a], part[a]);
