@@ -154,7 +154,7 @@ __global__ __launch_bounds__(256) void k_gemm_rows_mfma64(int m, int n, int k, i
   double ra[NQA], rb[NQB];
   auto fetch = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < NQA; q++) ra[q] = (arw[q] >= 0 && k0 + akk < kend) ? scale * A[arw[q] + k0 + akk] : 0.0;
+    for (int q = 0; q < NQA; q++) ra[q] = (arw[q] >= 0 && k0 + akk < kend) ? A[arw[q] + k0 + akk] : 0.0;
 #pragma unroll
     for (int q = 0; q < NQB; q++) {
       const int kb = bkk + BKS * q;
@@ -163,8 +163,10 @@ __global__ __launch_bounds__(256) void k_gemm_rows_mfma64(int m, int n, int k, i
   };
   if (kbeg < kend) fetch(kbeg);
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
+    // (the scale is applied here, when the loads have arrived anyway, so the
+    // prefetch of the next stage is not waited for before the MFMAs)
 #pragma unroll
-    for (int q = 0; q < NQA; q++) As[akk][ar0 + ARS * q] = ra[q];
+    for (int q = 0; q < NQA; q++) As[akk][ar0 + ARS * q] = scale * ra[q];
 #pragma unroll
     for (int q = 0; q < NQB; q++) Bs[bkk + BKS * q][bc] = rb[q];
     __syncthreads();
@@ -199,6 +201,8 @@ __global__ __launch_bounds__(256) void k_gemm_rows_mfma64(int m, int n, int k, i
         else if (BETA) C[crow[r] + cc] += acc[a][b][q];
         else C[crow[r] + cc] = acc[a][b][q];
       }
+      // one accumulator tile at a time through the VGPRs in the BETA epilogue
+      if (BETA) __builtin_amdgcn_sched_barrier(0);
     }
 }
 
@@ -222,13 +226,15 @@ struct GemmTile {
 };
 static GemmTile gemm_tile(int m) { return m >= 2048 ? GemmTile{128, 128} : GemmTile{64, 64}; }
 
-// split-K factor: enough workgroups to fill the chip (>= 1024), each split
-// keeping at least 64 of K (4 LDS stages)
+// split-K factor: enough workgroups for every resident slot of the chip
+// (256 CUs x 2 workgroups of 128 x 128, x 4 of 64 x 64), each split keeping
+// at least 64 of K (4 LDS stages)
 static int gemm_split(int m, int n, int k) {
   const GemmTile t = gemm_tile(m);
   const int tiles = ((n + t.bn - 1) / t.bn) * ((m + t.bm - 1) / t.bm);
+  const int target = t.bm == 128 ? 512 : 1024;
   int split = 1;
-  while (split < 64 && tiles * split < 1024 && k / (split * 2) >= 64) split *= 2;
+  while (split < 64 && tiles * split < target && k / (split * 2) >= 64) split *= 2;
   return split;
 }
 
@@ -257,18 +263,25 @@ static void launch_tile(int m, int n, int k, int split, bool part, const double*
   else if (beta) launch_mfma<BM, BN, true>(m, n, k, 1, A, arow, B, C, crow, alpha, nullptr, s);
   else launch_mfma<BM, BN, false>(m, n, k, 1, A, arow, B, C, crow, alpha, nullptr, s);
 }
+// (the 128 x 128 += epilogue is never launched: it would cost occupancy)
+template <>
+void launch_tile<128, 128>(int m, int n, int k, int split, bool part, const double* A, const long* arow,
+                           const double* B, double* C, const long* crow, double alpha, int beta, double* scratch,
+                           hipStream_t s) {
+  launch_mfma<128, 128, false>(m, n, k, split, A, arow, B, C, crow, alpha, part ? scratch : nullptr, s);
+}
 
 void launch_gemm_rows(bool cplx, int m, int n, int k, const double* A, const long* arow, const double* B,
                       double* C, const long* crow, double alpha, int beta, hipStream_t s, double* scratch,
                       size_t scratch_doubles) {
   if (m <= 0 || n <= 0) return;
   if (!cplx) {
-    // accumulating products (beta) also go through the partial buffer when
-    // there is one: the reduction then adds C
+    // an accumulating product without split-K uses the 64 x 64 tile, whose
+    // += epilogue keeps full occupancy
     int split = gemm_split(m, n, k);
-    const bool part = scratch && (split > 1 || beta) && size_t(split) * m * n <= scratch_doubles;
+    const bool part = scratch && split > 1 && size_t(split) * m * n <= scratch_doubles;
     if (!part) split = 1;
-    if (gemm_tile(m).bm == 128)
+    if (gemm_tile(m).bm == 128 && (part || !beta))
       launch_tile<128, 128>(m, n, k, split, part, A, arow, B, C, crow, alpha, beta, scratch, s);
     else
       launch_tile<64, 64>(m, n, k, split, part, A, arow, B, C, crow, alpha, beta, scratch, s);
