@@ -92,10 +92,11 @@ struct xfmm_context {
   bool have_ops = false;
   int nlevels = 0, nm2l = 0;
   DevBuf<double> uc2e_uT, uc2e_vT, dc2e_uT, dc2e_vT, m2mcat, l2lcat;
-  DevBuf<double> m2l64;
+  DevBuf<double> m2l64;  // M2L operators in the expanded MFMA form, [L][nfreq][26][64][4]
   DevBuf<float> m2l32;
   DevBuf<double> unit_surf;
   DevBuf<int> map_up, map_dn;
+  DevBuf<int> up_line_off, up_line_ent;  // input surface points grouped by x line
 
   // ---- tree
   bool have_tree = false;
@@ -171,6 +172,8 @@ struct xfmm_context {
     m.mixed_fft = f32_m2l_fft;
     m.map_up = map_up.ptr;
     m.map_dn = map_dn.ptr;
+    m.up_line_off = up_line_off.ptr;
+    m.up_line_ent = up_line_ent.ptr;
     m.fft_in = fft_in.ptr;
     m.fft_out = fft_out.ptr;
     return m;
@@ -369,7 +372,7 @@ void op_M2L(xfmm_context* c, bool timed) {
   if (timed) record(c, 16);
   launch_fft_up(m, t, c->m2l_src_node.ptr, c->n_m2l_src, c->stream);
   if (timed) record(c, 17);
-  const size_t opsz = size_t(NM2L) * c->nfreq * 64 * 2;
+  const size_t opsz = size_t(NM2L) * c->nfreq * 64 * 4;
   // one operator set (Laplace): a single launch over the blocks of all levels
   const int nlaunch = (c->nm2l == 1) ? 1 : c->depth;
   for (int l = 0; l < nlaunch; l++) {
@@ -469,6 +472,19 @@ int xfmm_create(int kernel, int p, double wavek_re, double wavek_im, int precisi
         seen[v] = 1;
       }
       (up ? c->map_up : c->map_dn).upload(m, c->stream);
+      if (up) {
+        // CSR of the input points by x line (y, z): line y p + z, entry (i << 4) | x
+        std::vector<int> off(size_t(p) * p + 1, 0), ent(m.size());
+        for (int v : m) off[(v / p % p) * p + v / (p * p) + 1]++;
+        for (size_t l = 0; l + 1 < off.size(); l++) off[l + 1] += off[l];
+        std::vector<int> pos(off.begin(), off.end() - 1);
+        for (size_t i = 0; i < m.size(); i++) {
+          const int v = m[i];
+          ent[pos[(v / p % p) * p + v / (p * p)]++] = int(i << 4) | (v % p);
+        }
+        c->up_line_off.upload(off, c->stream);
+        c->up_line_ent.upload(ent, c->stream);
+      }
     }
     M2LDev m = c->m2l_dev();
     const size_t lds = m2l_lds_bytes(m);
@@ -552,15 +568,26 @@ int xfmm_set_operators(xfmm_context* c, int nlevels, const double* uc2e_u, const
     c->m2mcat.upload(mm, c->stream);
     c->l2lcat.upload(ll, c->stream);
     const size_t msz = size_t(nm2l) * NM2L * c->nfreq * 64 * 2;
+    // reference layout uploaded, then expanded on the device into the per-lane
+    // MFMA operand form (twice the size) used by the Hadamard kernel
     if (c->f32_m2l) {
       std::vector<float> f(msz);
       for (size_t i = 0; i < msz; i++) f[i] = float(m2l[i]);
-      c->m2l32.upload(f, c->stream);
+      DevBuf<float> compact;
+      compact.upload(f, c->stream);
+      c->m2l32.alloc(2 * msz);
+      launch_m2l_expand(true, nm2l, c->nfreq, compact.ptr, c->m2l32.ptr, c->stream);
+      HIPCHK(hipStreamSynchronize(c->stream));
       c->m2l64.release();
     } else {
-      c->m2l64.upload(m2l, msz, c->stream);
+      DevBuf<double> compact;
+      compact.upload(m2l, msz, c->stream);
+      c->m2l64.alloc(2 * msz);
+      launch_m2l_expand(false, nm2l, c->nfreq, compact.ptr, c->m2l64.ptr, c->stream);
+      HIPCHK(hipStreamSynchronize(c->stream));
       c->m2l32.release();
     }
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     c->nlevels = nlevels;
     c->nm2l = nm2l;

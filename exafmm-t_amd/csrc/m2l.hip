@@ -196,143 +196,15 @@ __global__ __launch_bounds__(256) void k_ifft_dn(M2LDev m, DevTree t, const int*
 // [node][freq][child] layout.  Each 1-D DFT is register blocked: a thread loads
 // its p (or 2p) inputs once and produces all outputs of the line.
 
-template <typename RS, typename R, int P>
-__global__ __launch_bounds__(256) void k_fft_up_node(M2LDev m, DevTree t, const int* __restrict__ src_node,
-                                                     int nslots) {
-  extern __shared__ char smem_raw[];
-  constexpr int p = P, n1 = 2 * P, nh = P + 1, p3 = P * P * P;
-  const int slot = blockIdx.x;
-  if (slot >= nslots) return;
-  C2<R>* w = reinterpret_cast<C2<R>*>(smem_raw);  // n1
-  C2<R>* A = w + n1;                              // [c][z][y]   8 p^2
-  C2<R>* B = A + NCHILD * p * p;                  // [c][z][ky]  8 p n1
-  R* g = reinterpret_cast<R*>(B + NCHILD * p * n1);  // [c][z][y][x] 8 p^3
-  make_twiddles(w, n1, -1);
-  for (int e = threadIdx.x; e < NCHILD * p3; e += blockDim.x) g[e] = R(0);
-  __syncthreads();
-  const int node = src_node[slot];
-  const double* up = t.up + size_t(t.child0[node]) * m.nsurf;
-  for (int e = threadIdx.x; e < NCHILD * m.nsurf; e += blockDim.x) {
-    const int c = e / m.nsurf, i = e % m.nsurf;
-    g[c * p3 + m.map_up[i]] = R(up[e]);
-  }
-  __syncthreads();
-  C2<RS>* out = reinterpret_cast<C2<RS>*>(m.fft_in) + size_t(slot) * m.nfreq * NCHILD;
-  for (int kx = 0; kx < nh; kx++) {
-    // x pass: A[c][z][y] = sum_x g[c][z][y][x] W^(kx x)
-    for (int e = threadIdx.x; e < NCHILD * p * p; e += blockDim.x) {
-      const R* gl = g + e * p;
-      C2<R> acc{0, 0};
-#pragma unroll
-      for (int x = 0; x < p; x++) {
-        const C2<R> tw = w[(kx * x) % n1];
-        acc.x = fma(gl[x], tw.x, acc.x);
-        acc.y = fma(gl[x], tw.y, acc.y);
-      }
-      A[e] = acc;
-    }
-    __syncthreads();
-    // y pass: thread (c, z): B[c][z][ky] = sum_y A[c][z][y] W^(ky y)
-    for (int e = threadIdx.x; e < NCHILD * p; e += blockDim.x) {
-      C2<R> a[p];
-#pragma unroll
-      for (int y = 0; y < p; y++) a[y] = A[e * p + y];
-      for (int ky = 0; ky < n1; ky++) {
-        C2<R> acc{0, 0};
-#pragma unroll
-        for (int y = 0; y < p; y++) cfma(acc, a[y], w[(ky * y) % n1]);
-        B[e * n1 + ky] = acc;
-      }
-    }
-    __syncthreads();
-    // z pass: thread (c, ky) with c fastest; out[kx + nh (ky + n1 kz)][c]
-    for (int e = threadIdx.x; e < NCHILD * n1; e += blockDim.x) {
-      const int c = e & 7, ky = e >> 3;
-      C2<R> b[p];
-#pragma unroll
-      for (int z = 0; z < p; z++) b[z] = B[(c * p + z) * n1 + ky];
-      for (int kz = 0; kz < n1; kz++) {
-        C2<R> acc{0, 0};
-#pragma unroll
-        for (int z = 0; z < p; z++) cfma(acc, b[z], w[(kz * z) % n1]);
-        out[size_t(kx + nh * (ky + n1 * kz)) * NCHILD + c] = C2<RS>{RS(acc.x), RS(acc.y)};
-      }
-    }
-    __syncthreads();
-  }
-}
-
-template <typename RS, typename R, int P>
-__global__ __launch_bounds__(256) void k_ifft_dn_node(M2LDev m, DevTree t, const int* __restrict__ trg_node,
-                                                      const double* __restrict__ scale, int nslots) {
-  extern __shared__ char smem_raw[];
-  constexpr int p = P, n1 = 2 * P, nh = P + 1, p3 = P * P * P;
-  const int slot = blockIdx.x;
-  if (slot >= nslots) return;
-  C2<R>* w = reinterpret_cast<C2<R>*>(smem_raw);  // n1, exp(+2 pi i m / n1)
-  C2<R>* Bp = w + n1;                             // [c][zz][ky]  8 p n1
-  C2<R>* Ap = Bp + NCHILD * p * n1;               // [c][zz][yy]  8 p^2
-  R* G = reinterpret_cast<R*>(Ap + NCHILD * p * p);  // [c][zz][yy][xx] 8 p^3
-  make_twiddles(w, n1, +1);
-  for (int e = threadIdx.x; e < NCHILD * p3; e += blockDim.x) G[e] = R(0);
-  __syncthreads();
-  const C2<RS>* in = reinterpret_cast<const C2<RS>*>(m.fft_out) + size_t(slot) * m.nfreq * NCHILD;
-  for (int kx = 0; kx < nh; kx++) {
-    // inverse z pass, thread (c, ky): z' = p + zz
-    for (int e = threadIdx.x; e < NCHILD * n1; e += blockDim.x) {
-      const int c = e & 7, ky = e >> 3;
-      C2<R> v[n1];
-#pragma unroll
-      for (int kz = 0; kz < n1; kz++) {
-        const C2<RS> u = in[size_t(kx + nh * (ky + n1 * kz)) * NCHILD + c];
-        v[kz] = C2<R>{R(u.x), R(u.y)};
-      }
-      for (int zz = 0; zz < p; zz++) {
-        C2<R> acc{0, 0};
-#pragma unroll
-        for (int kz = 0; kz < n1; kz++) cfma(acc, v[kz], w[(kz * (p + zz)) % n1]);
-        Bp[(c * p + zz) * n1 + ky] = acc;
-      }
-    }
-    __syncthreads();
-    // inverse y pass, thread (c, zz): y' = p + yy
-    for (int e = threadIdx.x; e < NCHILD * p; e += blockDim.x) {
-      C2<R> v[n1];
-#pragma unroll
-      for (int ky = 0; ky < n1; ky++) v[ky] = Bp[e * n1 + ky];
-      for (int yy = 0; yy < p; yy++) {
-        C2<R> acc{0, 0};
-#pragma unroll
-        for (int ky = 0; ky < n1; ky++) cfma(acc, v[ky], w[(ky * (p + yy)) % n1]);
-        Ap[e * p + yy] = acc;
-      }
-    }
-    __syncthreads();
-    // x (c2r) accumulation: weight 1 for kx = 0 and the Nyquist plane, else 2
-    const R wt = (kx == 0 || kx == nh - 1) ? R(1) : R(2);
-    for (int e = threadIdx.x; e < NCHILD * p * p; e += blockDim.x) {
-      const C2<R> a = Ap[e];
-      R* Gl = G + e * p;
-#pragma unroll
-      for (int xx = 0; xx < p; xx++) {
-        const C2<R> tw = w[(kx * (p + xx)) % n1];
-        Gl[xx] += wt * (a.x * tw.x - a.y * tw.y);
-      }
-    }
-    __syncthreads();
-  }
-  const int node = trg_node[slot];
-  const double sc = scale[slot];
-  double* dn = t.dn + size_t(t.child0[node]) * m.nsurf;
-  for (int e = threadIdx.x; e < NCHILD * m.nsurf; e += blockDim.x) {
-    const int c = e / m.nsurf, i = e % m.nsurf;
-    dn[e] += sc * double(G[c * p3 + m.map_dn[i]]);
-  }
-}
-
+// LDS of the per-node MFMA transforms (rb = bytes of the compute type):
+// forward: twiddles, the 8 children's [0,p)^3 input grids, y-pass input
+// [2p][8p] and z-pass input [2p][16p]; inverse: twiddles, y-pass input
+// [4p][8p] and the y-pass output [p][p][8] complex.
 static size_t fft_node_lds(const M2LDev& m, size_t rb) {
   const size_t p = m.p, n1 = m.n1;
-  return n1 * 2 * rb + NCHILD * p * p * 2 * rb + NCHILD * p * n1 * 2 * rb + NCHILD * p * p * p * rb;
+  const size_t fwd = n1 * 2 * rb + NCHILD * p * p * p * rb + 2 * p * 8 * p * rb + 2 * p * 16 * p * rb;
+  const size_t inv = n1 * 2 * rb + 4 * p * 8 * p * rb + p * p * NCHILD * 2 * rb;
+  return fwd > inv ? fwd : inv;
 }
 
 // per-node transforms: real kernels, p in [3, 12], LDS within the CU's 160 KiB
@@ -406,25 +278,64 @@ struct Quad<double> {
   typedef double t __attribute__((ext_vector_type(4)));
 };
 
-// Per block of <= 64 target slots and a chunk of HAD_KCH frequencies.  For
-// each frequency the block stages the spectra of its source union and the 26
-// operator matrices in LDS, arranged so that every lane fetches its MFMA
-// operands for one relative position with one vector read each: children g
-// and g+4 sit side by side, {Re c_g, Im c_g, Re c_(g+4), Im c_(g+4)}.  The
-// next frequency's spectra and operators are loaded into registers while the
-// current one computes.  Each wave owns 16 targets; per relative position the
-// complex 8x8 product is one real 16x16x16 product (four 16x16x4 MFMAs):
-//   rows 0..7 of the result = Re of target children, rows 8..15 = Im,
-//   A[row][s] = Re/Im of M[s][row & 7], B[s][t] = Re/Im of the source spectrum.
-// A lane's union index for each relative position is fixed for the block and
-// held in registers; an absent source points at a zero row of the staging
-// array, so the 26 relative positions run without branches.
+// Operators in MFMA A-operand form: for operator level L, frequency k,
+// relative position r and lane (g, j) of a wave, the four A values of the
+// lane's four MFMAs (rows j < 8: Re of target child j, j >= 8: Im of child
+// j - 8; inner index g resp. g + 4 = source child):
+//   j <  8: { Re M[g][t],  Re M[g+4][t], -Im M[g][t], -Im M[g+4][t] }
+//   j >= 8: { Im M[g][t],  Im M[g+4][t],  Re M[g][t],  Re M[g+4][t] },  t = j & 7,
+// stored as X[((L nfreq + k) 26 + r) 64 + 16 g + j], so a wave reads one
+// contiguous 1 KiB per (k, r).  Built once from the reference layout
+// M[L][r][k][8 s + t] (complex).
 template <typename R>
-__global__ __launch_bounds__(256) void k_hadamard(M2LDev m, const R* __restrict__ ops, HadBlocks hb, int nchunk) {
+__global__ void k_m2l_expand(int nlev, int nfreq, const R* __restrict__ M, R* __restrict__ X) {
+  const size_t n = size_t(nlev) * nfreq * NM2L * 64;
+  for (size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += size_t(gridDim.x) * blockDim.x) {
+    const int lane = int(e % 64), g = lane >> 4, j = lane & 15, t = j & 7;
+    const size_t q = e / 64;
+    const int r = int(q % NM2L);
+    const size_t lk = q / NM2L;  // L * nfreq + k
+    const int k = int(lk % nfreq), L = int(lk / nfreq);
+    const R* src = M + ((size_t(L) * NM2L + r) * nfreq + k) * 128;
+    const R m0x = src[2 * (g * 8 + t)], m0y = src[2 * (g * 8 + t) + 1];
+    const R m1x = src[2 * ((g + 4) * 8 + t)], m1y = src[2 * ((g + 4) * 8 + t) + 1];
+    R* o = X + e * 4;
+    if (j < 8) { o[0] = m0x; o[1] = m1x; o[2] = -m0y; o[3] = -m1y; }
+    else { o[0] = m0y; o[1] = m1y; o[2] = m0x; o[3] = m1x; }
+  }
+}
+
+void launch_m2l_expand(bool f32, int nlev, int nfreq, const void* M, void* X, hipStream_t s) {
+  const size_t n = size_t(nlev) * nfreq * NM2L * 64;
+  const unsigned blocks = unsigned(std::min<size_t>((n + 255) / 256, 65536));
+  if (f32) hipLaunchKernelGGL(k_m2l_expand<float>, dim3(blocks), dim3(256), 0, s, nlev, nfreq, static_cast<const float*>(M), static_cast<float*>(X));
+  else hipLaunchKernelGGL(k_m2l_expand<double>, dim3(blocks), dim3(256), 0, s, nlev, nfreq, static_cast<const double*>(M), static_cast<double*>(X));
+}
+
+// Per block of <= 64 target slots and a chunk of HAD_KCH frequencies.  For
+// each frequency the block stages in LDS the spectra of its source union,
+// arranged so that every lane fetches its B operands for one relative
+// position with one 16-byte read: [g][union slot] = {Re c_g, Im c_g,
+// Re c_(g+4), Im c_(g+4)}, and the 26 operators in the expanded form above
+// (one 16-byte read per lane and relative position).  All staging moves
+// 16-byte vectors (no LDS bank conflicts on the stores), and the next
+// frequency's data is loaded into registers while the current one computes.
+// Each wave owns 16 targets; per
+// relative position the complex 8x8 product is one real 16x16x16 product
+// (four 16x16x4 MFMAs): rows 0..7 of the result = Re of target children,
+// rows 8..15 = Im.  A lane's union index for each relative position is fixed
+// for the block and held in registers; an absent source points at a zero
+// slot, so the 26 relative positions run without branches.
+template <typename R>
+__global__ __launch_bounds__(256) void k_hadamard(M2LDev m, const R* __restrict__ opsx, HadBlocks hb, int nchunk) {
   typedef typename MfmaT<R>::v4 v4;
   typedef typename Quad<R>::t q4;
-  __shared__ q4 Us[(HAD_UCAP + 1) * 4];  // [union slot][g]; slot HAD_UCAP stays zero
-  __shared__ q4 Ms[NM2L * 8 * 4];        // [r][target child][g]
+  constexpr int US = HAD_UCAP + 1;
+  // FP64 reads its operators straight from global memory (L1 / L2): staged,
+  // they would not leave room for a second workgroup per CU
+  constexpr bool STAGE_M = std::is_same<R, float>::value;
+  __shared__ q4 Us[4 * US];                   // [g][union slot]; slot HAD_UCAP stays zero
+  __shared__ q4 Ms[STAGE_M ? NM2L * 64 : 1];  // [r][lane]: A operands of the frequency
   // XCD-aware (chunk, block) mapping: XCD x = w % 8 owns chunks x, x+8, ...
   const int w = blockIdx.x;
   const int x = w & 7, i = w >> 3;
@@ -436,84 +347,77 @@ __global__ __launch_bounds__(256) void k_hadamard(M2LDev m, const R* __restrict_
   const int wave = tid >> 6;
   const int j = lane & 15, g = lane >> 4;
   const int nu = hb.nunion[b];
-  if (tid < 4) Us[HAD_UCAP * 4 + tid] = q4{0, 0, 0, 0};
+  if (tid < 4) Us[tid * US + HAD_UCAP] = q4{0, 0, 0, 0};
   int uoff[NM2L];
 #pragma unroll
   for (int r = 0; r < NM2L; r++) {
     const int u = hb.loc[(size_t(b) * NM2L + r) * 64 + wave * 16 + j];
-    uoff[r] = (u >= 0 ? u : HAD_UCAP) * 4 + g;
+    uoff[r] = g * US + (u >= 0 ? u : HAD_UCAP);
   }
   const C2<R>* fin = reinterpret_cast<const C2<R>*>(m.fft_in);
   C2<R>* fout = reinterpret_cast<C2<R>*>(m.fft_out);
-  const C2<R>* M = reinterpret_cast<const C2<R>*>(ops);
-  // staging assignment: spectra element e = tid + 256 q (q < 8) is child
-  // e % 8 of union slot e / 8; operator element e (q < 7) is entry e % 64 of
-  // relative position e / 64
-  constexpr int NU_Q = HAD_UCAP * NCHILD / 256, NM_Q = (NM2L * 64 + 255) / 256;
-  const int c8 = tid & 7;
+  const q4* X = reinterpret_cast<const q4*>(opsx);
+  // staging: element e = tid + 256 q (q < 4) is slot e / 4, pair g' = e % 4
+  // (children g' and g' + 4)
+  constexpr int NU_Q = HAD_UCAP * 4 / 256;
+  const int gp = tid & 3;
   int usrc[NU_Q];
 #pragma unroll
   for (int q = 0; q < NU_Q; q++) {
     const int e = tid + 256 * q;
-    usrc[q] = (e < nu * NCHILD) ? hb.uni[size_t(b) * HAD_UCAP + (e >> 3)] : -1;
+    usrc[q] = (e < nu * 4) ? hb.uni[size_t(b) * HAD_UCAP + (e >> 2)] : -1;
   }
-  C2<R> ru[NU_Q], rm[NM_Q];
+  constexpr int NM_Q = (NM2L * 64 + 255) / 256;
+  C2<R> ra[NU_Q], rb[NU_Q];
+  q4 rm[NM_Q];
   auto fetch = [&](int k) {
 #pragma unroll
     for (int q = 0; q < NU_Q; q++)
-      if (usrc[q] >= 0) ru[q] = fin[(size_t(usrc[q]) * m.nfreq + k) * NCHILD + c8];
+      if (usrc[q] >= 0) {
+        const C2<R>* f = fin + (size_t(usrc[q]) * m.nfreq + k) * NCHILD;
+        ra[q] = f[gp];
+        rb[q] = f[gp + 4];
+      }
+    if constexpr (STAGE_M) {
+      const q4* xk = X + size_t(k) * NM2L * 64;
 #pragma unroll
-    for (int q = 0; q < NM_Q; q++) {
-      const int e = tid + 256 * q;
-      if (e < NM2L * 64) rm[q] = M[(size_t(e >> 6) * m.nfreq + k) * 64 + (e & 63)];
+      for (int q = 0; q < NM_Q; q++)
+        if (tid + 256 * q < NM2L * 64) rm[q] = xk[tid + 256 * q];
     }
   };
-  R* Usr = reinterpret_cast<R*>(Us);
-  R* Msr = reinterpret_cast<R*>(Ms);
-  const int cpos = 4 * (c8 & 3) + 2 * (c8 >> 2);
-  const int j1 = j & 7;
   const bool im_out = j >= 8;
+  (void)im_out;
   const int trg = hb.trg[size_t(b) * 64 + wave * 16 + j];
   fetch(kbeg);
   for (int k = kbeg; k < kend; k++) {
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < NU_Q; q++)
-      if (usrc[q] >= 0) {
-        const int e = tid + 256 * q;
-        Usr[(e >> 3) * 16 + cpos] = ru[q].x;
-        Usr[(e >> 3) * 16 + cpos + 1] = ru[q].y;
-      }
+      if (usrc[q] >= 0) Us[gp * US + ((tid + 256 * q) >> 2)] = q4{ra[q].x, ra[q].y, rb[q].x, rb[q].y};
+    if constexpr (STAGE_M) {
 #pragma unroll
-    for (int q = 0; q < NM_Q; q++) {
-      const int e = tid + 256 * q;
-      if (e < NM2L * 64) {
-        // entry (s, t) = operator from source child s to target child t
-        const int r = e >> 6, s = (e >> 3) & 7, t = e & 7;
-        const int o = ((r * 8 + t) * 4 + (s & 3)) * 4 + 2 * (s >> 2);
-        Msr[o] = rm[q].x;
-        Msr[o + 1] = rm[q].y;
-      }
+      for (int q = 0; q < NM_Q; q++)
+        if (tid + 256 * q < NM2L * 64) Ms[tid + 256 * q] = rm[q];
     }
     __syncthreads();
     if (k + 1 < kend) fetch(k + 1);
-    // two accumulators (even / odd r) and operands of r + 1 read while r
-    // computes
+    // two accumulators (even / odd r); operands of r + 1 read while r computes
     v4 acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
-    q4 mm = Ms[j1 * 4 + g];
+    const q4* Ma = STAGE_M ? Ms : X + size_t(k) * NM2L * 64;
+    q4 mm = Ma[lane];
     q4 vv = Us[uoff[0]];
 #pragma unroll
     for (int r = 0; r < NM2L; r++) {
       q4 mn = mm, vn = vv;
       if (r + 1 < NM2L) {
-        mn = Ms[((r + 1) * 8 + j1) * 4 + g];
+        mn = Ma[(r + 1) * 64 + lane];
         vn = Us[uoff[r + 1]];
       }
       v4& acc = (r & 1) ? acc1 : acc0;
-      acc = MfmaT<R>::mma(im_out ? mm.y : mm.x, vv.x, acc);
-      acc = MfmaT<R>::mma(im_out ? mm.w : mm.z, vv.z, acc);
-      acc = MfmaT<R>::mma(im_out ? mm.x : -mm.y, vv.y, acc);
-      acc = MfmaT<R>::mma(im_out ? mm.z : -mm.w, vv.w, acc);
+      acc = MfmaT<R>::mma(mm.x, vv.x, acc);
+      acc = MfmaT<R>::mma(mm.y, vv.z, acc);
+      acc = MfmaT<R>::mma(mm.z, vv.y, acc);
+      acc = MfmaT<R>::mma(mm.w, vv.w, acc);
       mm = mn;
       vv = vn;
     }
@@ -529,6 +433,299 @@ __global__ __launch_bounds__(256) void k_hadamard(M2LDev m, const R* __restrict_
       }
     }
   }
+}
+
+// ------------------------------------------------ per-node MFMA transforms
+//
+// One workgroup per M2L node (its 8 children), in kx slabs.  Each 1-D DFT
+// restricted to the index ranges that matter (input [0,p), output [0,2p) for
+// the forward transform; input [0,2p), output [p,2p) for the inverse) is a
+// real matrix product: rows = (output frequency or point, re/im), inner =
+// (input index, re/im), columns = (other index, child).  The DFT matrix is
+// the MFMA A operand and stays in registers for the whole kernel; B comes
+// from LDS (or, for the inverse z pass, straight from the spectra).  The x
+// direction is done on the VALU, sparsely: forward, only the surface points
+// of each x line contribute (line table); inverse, the c2r sum is evaluated
+// only at the surface points.
+//
+// Row order inside a 16-row tile: chosen so that a lane's result registers
+// hold the (re, im) pair of one output, RowMap::fa/ra for the A operand rows
+// and RowMap::fd for the result registers (q & 1 = re/im).
+template <typename R>
+struct RowMap;
+template <>
+struct RowMap<float> {  // v_mfma_f32_16x16x4_f32: register q of lane group g is row 4g + q
+  static __device__ __forceinline__ int fa(int j) { return j >> 1; }
+  static __device__ __forceinline__ int ra(int j) { return j & 1; }
+  static __device__ __forceinline__ int fd(int g, int q) { return 2 * g + (q >> 1); }
+};
+template <>
+struct RowMap<double> {  // v_mfma_f64_16x16x4_f64: register q of lane group g is row g + 4q
+  static __device__ __forceinline__ int fa(int j) { return (j & 3) + 4 * (j >> 3); }
+  static __device__ __forceinline__ int ra(int j) { return (j >> 2) & 1; }
+  static __device__ __forceinline__ int fd(int g, int q) { return g + 4 * (q >> 1); }
+};
+
+// A-operand entry of the real form of a complex DFT matrix: output (f, ro),
+// input (v, ri), phase sign * 2 pi * ((f' v') mod n1) / n1.
+__device__ __forceinline__ double dft_entry(int ph, int n1, int sign, int ro, int ri) {
+  double sn, cs;
+  sincospi(double(sign) * 2.0 * double(ph % n1) / double(n1), &sn, &cs);
+  // (re, im) out = (c re - s im, s re + c im)
+  return ro == 0 ? (ri == 0 ? cs : -sn) : (ri == 0 ? sn : cs);
+}
+
+// fft_up_equiv for one node: out[(kx + nh (ky + n1 kz)) * 8 + c] = sum over
+// the child's surface points (x,y,z) of up[c][i] exp(-2 pi i (kx x + ky y +
+// kz z) / n1)  (reference fmm_scale_invariant.h:560-604).
+template <typename RS, typename R, int P>
+__global__ __launch_bounds__(256) void k_fft_up_mfma(M2LDev m, DevTree t, const int* __restrict__ src_node,
+                                                     int nslots) {
+  typedef typename MfmaT<R>::v4 v4;
+  constexpr int n1 = 2 * P, nh = P + 1;
+  constexpr int KK = 2 * P, KS = (KK + 3) / 4;      // inner: (y or z, re/im)
+  constexpr int MT = (2 * n1 + 15) / 16;             // rows: (ky or kz, re/im)
+  constexpr int NCY = 8 * P, NTY = (NCY + 15) / 16;  // y pass columns (z, c)
+  constexpr int NCZ = 8 * n1, NTZ = NCZ / 16;        // z pass columns (ky, c)
+  extern __shared__ char smem_raw[];
+  const int slot = blockIdx.x;
+  if (slot >= nslots) return;
+  const int ns = m.nsurf;
+  constexpr int P3 = P * P * P;
+  C2<R>* w = reinterpret_cast<C2<R>*>(smem_raw);  // exp(-2 pi i m / n1)
+  R* gx = reinterpret_cast<R*>(w + n1);           // [c][z][y][x] input grid
+  R* Ay = gx + NCHILD * P3;                       // [(y, re/im)][(z, c)]
+  R* Bz = Ay + KK * NCY;                          // [(z, re/im)][(ky, c)]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int j = lane & 15, g = lane >> 4;
+  make_twiddles(w, n1, -1);
+  for (int e = tid; e < NCHILD * P3; e += 256) gx[e] = R(0);
+  __syncthreads();
+  const int node = src_node[slot];
+  const double* up = t.up + size_t(t.child0[node]) * ns;
+  for (int e = tid; e < NCHILD * ns; e += 256) {
+    const int c = e / ns, i = e - c * ns;
+    gx[c * P3 + m.map_up[i]] = R(up[e]);
+  }
+  R wf[MT][KS];
+#pragma unroll
+  for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
+      const int f = mt * 8 + RowMap<R>::fa(j), k = ks * 4 + g, v = k >> 1;
+      wf[mt][ks] = (f < n1 && k < KK) ? R(dft_entry(f * v, n1, -1, RowMap<R>::ra(j), k & 1)) : R(0);
+    }
+  __syncthreads();
+  C2<RS>* out = reinterpret_cast<C2<RS>*>(m.fft_in) + size_t(slot) * m.nfreq * NCHILD;
+  for (int kx = 0; kx < nh; kx++) {
+    // x: Ay[(y, re/im)][(z, c)] = sum_x gx[c][z][y][x] exp(-2 pi i kx x / n1)
+    C2<R> tw[P];
+#pragma unroll
+    for (int x = 0; x < P; x++) tw[x] = w[(kx * x) % n1];
+    for (int e = tid; e < NCHILD * P * P; e += 256) {
+      const int c = e & 7, line = e >> 3, y = line / P, z = line % P;
+      const R* gl = gx + c * P3 + (z * P + y) * P;
+      C2<R> acc{0, 0};
+#pragma unroll
+      for (int x = 0; x < P; x++) {
+        acc.x = fma(gl[x], tw[x].x, acc.x);
+        acc.y = fma(gl[x], tw[x].y, acc.y);
+      }
+      Ay[(2 * y) * NCY + z * 8 + c] = acc.x;
+      Ay[(2 * y + 1) * NCY + z * 8 + c] = acc.y;
+    }
+    __syncthreads();
+    // y: Bz[(z, re/im)][(ky, c)] = sum_y W(ky, y) Ay[(y, .)][(z, c)]
+    for (int nt = wave; nt < NTY; nt += 4) {
+      const int col = nt * 16 + j;
+      R b[KS];
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++) {
+        const int k = ks * 4 + g;
+        b[ks] = (k < KK && col < NCY) ? Ay[k * NCY + col] : R(0);
+      }
+      const int z = col >> 3, c = col & 7;
+#pragma unroll
+      for (int mt = 0; mt < MT; mt++) {
+        v4 acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) acc = MfmaT<R>::mma(wf[mt][ks], b[ks], acc);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int ky = mt * 8 + RowMap<R>::fd(g, q);
+          if (ky < n1 && col < NCY) Bz[(2 * z + (q & 1)) * NCZ + ky * 8 + c] = acc[q];
+        }
+      }
+    }
+    __syncthreads();
+    // z: out(kx, ky, kz)[c] = sum_z W(kz, z) Bz[(z, .)][(ky, c)]
+    for (int nt = wave; nt < NTZ; nt += 4) {
+      const int col = nt * 16 + j, ky = col >> 3, c = col & 7;
+      R b[KS];
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++) {
+        const int k = ks * 4 + g;
+        b[ks] = (k < KK) ? Bz[k * NCZ + col] : R(0);
+      }
+#pragma unroll
+      for (int mt = 0; mt < MT; mt++) {
+        v4 acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) acc = MfmaT<R>::mma(wf[mt][ks], b[ks], acc);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          const int kz = mt * 8 + RowMap<R>::fd(g, 2 * h);
+          if (kz < n1) out[size_t(kx + nh * (ky + n1 * kz)) * NCHILD + c] = C2<RS>{RS(acc[2 * h]), RS(acc[2 * h + 1])};
+        }
+      }
+    }
+    // the next x pass writes Ay only: its last readers (y pass) are behind
+    // the barrier above
+  }
+}
+
+// ifft_dn_check for one node: dn[c][i] += scale * sum over (kx, ky, kz) of
+// w(kx) Re(in(kx,ky,kz)[c] exp(+2 pi i (kx x + ky y + kz z) / n1)) at the
+// child's check-surface points (x,y,z) in [p,2p)^3, w = 1 for kx = 0 and the
+// Nyquist plane, else 2 (c2r of the half spectrum; reference
+// fmm_scale_invariant.h:606-646).
+template <typename RS, typename R, int P>
+__global__ __launch_bounds__(256) void k_ifft_dn_mfma(M2LDev m, DevTree t, const int* __restrict__ trg_node,
+                                                      const double* __restrict__ scale, int nslots) {
+  typedef typename MfmaT<R>::v4 v4;
+  constexpr int n1 = 2 * P, nh = P + 1;
+  constexpr int KK = 2 * n1, KS = KK / 4;            // inner: (kz or ky, re/im)
+  constexpr int MT = (2 * P + 15) / 16;              // rows: (z or y in [p,2p), re/im)
+  constexpr int NCZ = 8 * n1, NTZ = NCZ / 16;        // z pass columns (ky, c)
+  constexpr int NTZW = (NTZ + 3) / 4;                // z tiles per wave (at most)
+  constexpr int NCY = 8 * P, NTY = (NCY + 15) / 16;  // y pass columns (z, c)
+  constexpr int NSMAX = 6 * (P - 1) * (P - 1) + 2;   // surface points
+  constexpr int NG = (NCHILD * NSMAX + 255) / 256;   // surface values per thread
+  extern __shared__ char smem_raw[];
+  const int slot = blockIdx.x;
+  if (slot >= nslots) return;
+  const int ns = m.nsurf;
+  C2<R>* w = reinterpret_cast<C2<R>*>(smem_raw);   // exp(+2 pi i m / n1)
+  R* Bp = reinterpret_cast<R*>(w + n1);            // [(ky, re/im)][(z, c)]
+  C2<R>* Ap = reinterpret_cast<C2<R>*>(Bp + KK * NCY);  // [z][y][c]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int j = lane & 15, g = lane >> 4;
+  make_twiddles(w, n1, +1);
+  R wf[MT][KS];
+#pragma unroll
+  for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
+      const int f = mt * 8 + RowMap<R>::fa(j), k = ks * 4 + g, v = k >> 1;
+      wf[mt][ks] = (f < P) ? R(dft_entry(v * (P + f), n1, +1, RowMap<R>::ra(j), k & 1)) : R(0);
+    }
+  // surface results of this thread, element e = tid + 256 r = (c, i), and
+  // packed per element: bits 0-11 the y-pass output it reads, 12-16 its x
+  // step p + x, 17-21 the running phase kx (p + x) mod n1 (-1: no element)
+  static_assert(P * P * NCHILD < 4096 && 2 * P < 32, "packing");
+  R gacc[NG];
+  int st[NG];
+#pragma unroll
+  for (int r = 0; r < NG; r++) {
+    const int e = tid + 256 * r;
+    gacc[r] = R(0);
+    st[r] = -1;
+    if (e < NCHILD * ns) {
+      const int c = e / ns, i = e - c * ns, gi = m.map_dn[i];
+      const int x = gi % P, y = (gi / P) % P, z = gi / (P * P);
+      st[r] = ((z * P + y) * NCHILD + c) | ((P + x) << 12);
+    }
+  }
+  const C2<RS>* in = reinterpret_cast<const C2<RS>*>(m.fft_out) + size_t(slot) * m.nfreq * NCHILD;
+  // z-pass input fragments of this wave's tiles, loaded one slab ahead
+  RS bz[NTZW][KS];
+  auto fetch = [&](int kx) {
+#pragma unroll
+    for (int s = 0; s < NTZW; s++) {
+      const int nt = wave + 4 * s;
+      const int col = nt * 16 + j, ky = col >> 3, c = col & 7;
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++) {
+        const int k = ks * 4 + g;
+        bz[s][ks] = RS(0);
+        if (nt < NTZ) {
+          const RS* src = reinterpret_cast<const RS*>(in + size_t(kx + nh * (ky + n1 * (k >> 1))) * NCHILD + c);
+          bz[s][ks] = src[k & 1];
+        }
+      }
+    }
+  };
+  fetch(0);
+  __syncthreads();
+  for (int kx = 0; kx < nh; kx++) {
+    // z: Bp[(ky, re/im)][(z, c)] = sum_kz W(z, kz) in(kx, ky, kz)[c]
+#pragma unroll
+    for (int s = 0; s < NTZW; s++) {
+      const int nt = wave + 4 * s;
+      if (nt >= NTZ) continue;
+      const int col = nt * 16 + j, ky = col >> 3, c = col & 7;
+      R b[KS];
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++) b[ks] = R(bz[s][ks]);
+#pragma unroll
+      for (int mt = 0; mt < MT; mt++) {
+        v4 acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) acc = MfmaT<R>::mma(wf[mt][ks], b[ks], acc);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int z = mt * 8 + RowMap<R>::fd(g, q);
+          if (z < P) Bp[(2 * ky + (q & 1)) * NCY + z * 8 + c] = acc[q];
+        }
+      }
+    }
+    if (kx + 1 < nh) fetch(kx + 1);
+    __syncthreads();
+    // y: Ap[z][y][c] = sum_ky W(y, ky) Bp[(ky, .)][(z, c)]
+    for (int nt = wave; nt < NTY; nt += 4) {
+      const int col = nt * 16 + j;
+      R b[KS];
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++) b[ks] = (col < NCY) ? Bp[(ks * 4 + g) * NCY + col] : R(0);
+      const int z = col >> 3, c = col & 7;
+#pragma unroll
+      for (int mt = 0; mt < MT; mt++) {
+        v4 acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) acc = MfmaT<R>::mma(wf[mt][ks], b[ks], acc);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          const int y = mt * 8 + RowMap<R>::fd(g, 2 * h);
+          if (y < P && col < NCY) Ap[(z * P + y) * NCHILD + c] = C2<R>{acc[2 * h], acc[2 * h + 1]};
+        }
+      }
+    }
+    __syncthreads();
+    // x (c2r) at the check-surface points, weight 1 for kx = 0 and the
+    // Nyquist plane, else 2
+    const R wt = (kx == 0 || kx == nh - 1) ? R(1) : R(2);
+#pragma unroll
+    for (int r = 0; r < NG; r++) {
+      if (st[r] >= 0) {
+        const C2<R> a = Ap[st[r] & 4095];
+        int ph = st[r] >> 17;
+        const C2<R> tw = w[ph];
+        gacc[r] += wt * (a.x * tw.x - a.y * tw.y);
+        ph += (st[r] >> 12) & 31;
+        ph -= (ph >= n1) ? n1 : 0;
+        ph -= (ph >= n1) ? n1 : 0;
+        st[r] = (st[r] & 131071) | (ph << 17);
+      }
+    }
+    // the next z pass writes Bp only (its readers are behind the barrier
+    // above); Ap is next written after the following barrier
+  }
+  const int node = trg_node[slot];
+  const double sc = scale[slot];
+  double* dn = t.dn + size_t(t.child0[node]) * ns;
+#pragma unroll
+  for (int r = 0; r < NG; r++)
+    if (st[r] >= 0) dn[tid + 256 * r] += sc * double(gacc[r]);
 }
 
 // --------------------------------------------------------------- launchers
@@ -572,14 +769,14 @@ void launch_fft_up(const M2LDev& m, const DevTree& t, const int* src_node, int n
     node_dispatch(m.p, [&](auto P) {
       constexpr int PP = decltype(P)::value;
       if (m.mixed && m.mixed_fft) {
-        allow_lds(k_fft_up_node<float, float, PP>, lds);
-        hipLaunchKernelGGL((k_fft_up_node<float, float, PP>), dim3(nslots), dim3(256), lds, s, m, t, src_node, nslots);
+        allow_lds(k_fft_up_mfma<float, float, PP>, lds);
+        hipLaunchKernelGGL((k_fft_up_mfma<float, float, PP>), dim3(nslots), dim3(256), lds, s, m, t, src_node, nslots);
       } else if (m.mixed) {
-        allow_lds(k_fft_up_node<float, double, PP>, lds);
-        hipLaunchKernelGGL((k_fft_up_node<float, double, PP>), dim3(nslots), dim3(256), lds, s, m, t, src_node, nslots);
+        allow_lds(k_fft_up_mfma<float, double, PP>, lds);
+        hipLaunchKernelGGL((k_fft_up_mfma<float, double, PP>), dim3(nslots), dim3(256), lds, s, m, t, src_node, nslots);
       } else {
-        allow_lds(k_fft_up_node<double, double, PP>, lds);
-        hipLaunchKernelGGL((k_fft_up_node<double, double, PP>), dim3(nslots), dim3(256), lds, s, m, t, src_node, nslots);
+        allow_lds(k_fft_up_mfma<double, double, PP>, lds);
+        hipLaunchKernelGGL((k_fft_up_mfma<double, double, PP>), dim3(nslots), dim3(256), lds, s, m, t, src_node, nslots);
       }
     });
     return;
@@ -605,16 +802,16 @@ void launch_ifft_dn(const M2LDev& m, const DevTree& t, const int* trg_node, cons
     node_dispatch(m.p, [&](auto P) {
       constexpr int PP = decltype(P)::value;
       if (m.mixed && m.mixed_fft) {
-        allow_lds(k_ifft_dn_node<float, float, PP>, lds);
-        hipLaunchKernelGGL((k_ifft_dn_node<float, float, PP>), dim3(nslots), dim3(256), lds, s, m, t, trg_node, scale,
+        allow_lds(k_ifft_dn_mfma<float, float, PP>, lds);
+        hipLaunchKernelGGL((k_ifft_dn_mfma<float, float, PP>), dim3(nslots), dim3(256), lds, s, m, t, trg_node, scale,
                            nslots);
       } else if (m.mixed) {
-        allow_lds(k_ifft_dn_node<float, double, PP>, lds);
-        hipLaunchKernelGGL((k_ifft_dn_node<float, double, PP>), dim3(nslots), dim3(256), lds, s, m, t, trg_node, scale,
+        allow_lds(k_ifft_dn_mfma<float, double, PP>, lds);
+        hipLaunchKernelGGL((k_ifft_dn_mfma<float, double, PP>), dim3(nslots), dim3(256), lds, s, m, t, trg_node, scale,
                            nslots);
       } else {
-        allow_lds(k_ifft_dn_node<double, double, PP>, lds);
-        hipLaunchKernelGGL((k_ifft_dn_node<double, double, PP>), dim3(nslots), dim3(256), lds, s, m, t, trg_node,
+        allow_lds(k_ifft_dn_mfma<double, double, PP>, lds);
+        hipLaunchKernelGGL((k_ifft_dn_mfma<double, double, PP>), dim3(nslots), dim3(256), lds, s, m, t, trg_node,
                            scale, nslots);
       }
     });
@@ -633,15 +830,15 @@ void launch_ifft_dn(const M2LDev& m, const DevTree& t, const int* trg_node, cons
   }
 }
 
-void launch_hadamard(const M2LDev& m, const void* m2l_ops, const HadBlocks& hb, hipStream_t s) {
+void launch_hadamard(const M2LDev& m, const void* m2l_opsx, const HadBlocks& hb, hipStream_t s) {
   if (hb.nblk <= 0) return;
   const int nchunk = (m.nfreq + HAD_KCH - 1) / HAD_KCH;
   const int nchunk8 = (nchunk + 7) / 8 * 8;
   dim3 grid(unsigned(nchunk8) * unsigned(hb.nblk));
   if (m.mixed)
-    hipLaunchKernelGGL((k_hadamard<float>), grid, dim3(256), 0, s, m, static_cast<const float*>(m2l_ops), hb, nchunk);
+    hipLaunchKernelGGL((k_hadamard<float>), grid, dim3(256), 0, s, m, static_cast<const float*>(m2l_opsx), hb, nchunk);
   else
-    hipLaunchKernelGGL((k_hadamard<double>), grid, dim3(256), 0, s, m, static_cast<const double*>(m2l_ops), hb, nchunk);
+    hipLaunchKernelGGL((k_hadamard<double>), grid, dim3(256), 0, s, m, static_cast<const double*>(m2l_opsx), hb, nchunk);
 }
 
 }  // namespace xfmm
