@@ -118,6 +118,22 @@ def cpu_baseline(p, ncrit, n_sample):
                       % (n_sample, p, ncrit, dt)}
 
 
+def measured_traffic(dom, mixed, a):
+    """HBM bytes per launch of the dominant kernel from the committed FETCH_SIZE / WRITE_SIZE
+    measurement of the same workload (profiles/), scaled by the copy-kernel calibration; None when
+    no measurement matches this kernel and workload."""
+    path = os.path.join(ROOT, "profiles", "laplace_N%s_p%d_hadamard_traffic.json" % (
+        "2M" if a.n == 2_000_000 else str(a.n), a.p))
+    if dom != "HAD" or a.ncrit != 128 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    if t.get("kernel") != "k_hadamard<%s>" % ("float" if mixed else "double"):
+        return None
+    c = t["calibration"]
+    return 1024.0 * (t["fetch_kib_per_dispatch"] * c["fetch_scale"] + t["write_kib_per_dispatch"] * c["write_scale"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,6 +214,7 @@ def main():
                 "algorithmic": "%d nodes x %d freq x 8 children complex" % (nslots, nfreq)}
     roof["frac"] = roof["achieved"] / roof["peak"]
     roof["ms"] = cand[dom]
+    roof["traffic"] = measured_traffic(dom, mixed, a)
 
     line = {
         "metric": METRIC,
