@@ -9,6 +9,12 @@ FP64 path is timed beside it.  Errors are relative L2 against an FP64 direct
 sum on 2,000 seeded sample targets computed by the engine's direct-sum kernel.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--n 2000000] [--no-cpu-baseline]
+
+The other workloads of the same evaluation are measured with --workload (one JSON line each, not the
+headline): laplace_small (N=100k, p=8, ncrit=64, charges U[0,1)), yukawa_sphere (k=5, N=1M on the
+unit sphere, charges N(0,1), p=8, ncrit=64), helmholtz_cube (k=20, N=4M uniform, complex charges,
+p=10, ncrit=128) and laplace_batched (64 Plummer problems, N log-uniform in [1e4, 5e5], charges 1/N,
+p=8, ncrit=64; every problem's evaluation captured in a HIP graph, all graphs replayed back to back).
 """
 import argparse
 import json
@@ -134,6 +140,172 @@ def measured_traffic(dom, mixed, a):
     return 1024.0 * (t["fetch_kib_per_dispatch"] * c["fetch_scale"] + t["write_kib_per_dispatch"] * c["write_scale"])
 
 
+# ------------------------------------------------------------------ other workloads
+
+def make_points(dist, n, rng):
+    if dist == "cube":
+        return rng.random((n, 3))
+    if dist == "sphere":  # normalised Gaussian samples on the unit sphere
+        x = rng.normal(size=(n, 3))
+        return x / np.linalg.norm(x, axis=1)[:, None]
+    if dist == "plummer":  # scale radius 1, radius truncated at 10
+        m = rng.random(n) * 0.999
+        r = np.minimum(1.0 / np.sqrt(m ** (-2.0 / 3.0) - 1.0), 10.0)
+        v = rng.normal(size=(n, 3))
+        return v / np.linalg.norm(v, axis=1)[:, None] * r[:, None]
+    raise ValueError(dist)
+
+
+WORKLOADS = {
+    # name: kernel id, N, p, ncrit, wavek, distribution, charges
+    "laplace_small": (0, 100_000, 8, 64, 0.0, "cube", "u01"),
+    "yukawa_sphere": (2, 1_000_000, 8, 64, 5.0, "sphere", "normal"),
+    "helmholtz_cube": (1, 4_000_000, 10, 128, 20.0, "cube", "complex"),
+}
+
+
+def setup_generic(kid, n, p, ncrit, wavek, dist, charges, seed):
+    import exafmm
+    mod = [exafmm.laplace, exafmm.helmholtz, exafmm.modified_helmholtz][kid]
+    rng = np.random.default_rng(seed)
+    x = make_points(dist, n, rng)
+    if charges == "u01":
+        q = rng.random(n)
+    elif charges == "normal":
+        q = rng.normal(size=n)
+    elif charges == "complex":
+        q = rng.uniform(-1.0, 1.0, n) + 1j * rng.uniform(-1.0, 1.0, n)
+    else:
+        q = np.full(n, float(charges))
+    fname = os.path.join(tempfile.gettempdir(), "bench_k%d_p%d_w%g.dat" % (kid, p, wavek))
+    if kid == 0:
+        fmm = mod.LaplaceFmm(p, ncrit, fname)
+    elif kid == 1:
+        fmm = mod.HelmholtzFmm(p, ncrit, complex(wavek), fname)
+    else:
+        fmm = mod.ModifiedHelmholtzFmm(p, ncrit, wavek, fname)
+    t0 = time.perf_counter()
+    tree = mod.setup(mod.init_sources(x, q), mod.init_targets(x), fmm)
+    flat = fmm.flat_tree(tree)
+    ops = fmm.operators()
+    qs = fmm.sorted_charges(tree)
+    return dict(kid=kid, p=p, wavek=wavek, flat=flat, ops=ops, qs=qs, depth=fmm.depth, n=n,
+                t_setup=time.perf_counter() - t0)
+
+
+def engine_for(prob, precision):
+    from exafmm import abi
+    eng = abi.Engine(prob["kid"], prob["p"], prob["wavek"], precision, 0)
+    eng.set_operators(prob["ops"])
+    eng.set_tree(prob["flat"])
+    eng.set_charges(prob["qs"])
+    return eng
+
+
+def sample_error_generic(prob, values, nsample, seed):
+    from exafmm import abi
+    flat = prob["flat"]
+    tc = flat["trg_coord"].reshape(-1, 3)
+    rng = np.random.default_rng(seed)
+    idx = np.sort(rng.choice(tc.shape[0], size=min(nsample, tc.shape[0]), replace=False))
+    ref = abi.direct(prob["kid"], flat["src_coord"].reshape(-1, 3), prob["qs"], tc[idx], prob["wavek"], abi.FP64,
+                     True)
+    v = values[idx]
+    ep = float(np.linalg.norm(v[:, 0] - ref[:, 0]) / np.linalg.norm(ref[:, 0]))
+    eg = float(np.linalg.norm(v[:, 1:] - ref[:, 1:]) / np.linalg.norm(ref[:, 1:]))
+    return ep, eg
+
+
+def run_single_workload(a, name):
+    from exafmm import abi
+    kid, n, p, ncrit, wavek, dist, charges = WORKLOADS[name]
+    log("setup %s: N=%d p=%d ncrit=%d" % (name, n, p, ncrit))
+    prob = setup_generic(kid, n, p, ncrit, wavek, dist, charges, seed=0)
+    log("setup %.1fs depth %d nodes %d" % (prob["t_setup"], prob["depth"], prob["flat"]["nnodes"]))
+    res = {}
+    for pname, prec in (("mixed", abi.MIXED), ("fp64", abi.FP64)):
+        eng = engine_for(prob, prec)
+        for _ in range(a.warmup):
+            eng.upward_pass()
+            eng.downward_pass()
+        eng.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            eng.upward_pass()
+            eng.downward_pass()
+        eng.synchronize()
+        dt = (time.perf_counter() - t0) / a.steps
+        kt = eng.timings()
+        err = sample_error_generic(prob, eng.values(), 1000, seed=123)
+        eng.close()
+        res[pname] = dict(ms=1e3 * dt, value=n / dt, err=err, kernels={k: round(v, 4) for k, v in kt.items()})
+        log("%s %s: %.3f ms/step err phi %.2e grad %.2e" % (name, pname, 1e3 * dt, err[0], err[1]))
+    m, f = res["mixed"], res["fp64"]
+    return {
+        "metric": "particles/s, %s FMM evaluation" % name, "value": m["value"], "unit": "particles/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": m["ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 P2P/M2L + f64 pinv solves" + (" (complex)" if kid == 1 else ""),
+        "data": "synthetic (seeded %s, charges %s)" % (dist, charges),
+        "config": {"workload": "%s_N%d_p%d_ncrit%d" % (name, n, p, ncrit), "kernel": ["laplace", "helmholtz",
+                   "modified_helmholtz"][kid], "wavek": wavek, "N": n, "p": p, "ncrit": ncrit,
+                   "depth": prob["depth"], "nodes": int(prob["flat"]["nnodes"])},
+        "err_phi": m["err"][0], "err_grad": m["err"][1], "kernels_ms": m["kernels"],
+        "fp64_value": f["value"], "fp64_ms_per_step": f["ms"], "fp64_err_phi": f["err"][0],
+        "fp64_err_grad": f["err"][1], "fp64_kernels_ms": f["kernels"], "mixed_vs_fp64": m["value"] / f["value"],
+        "setup_s": round(prob["t_setup"], 2), "roofline": None, "cpu_baseline": None,
+    }
+
+
+def run_batched(a, nprob=64):
+    """64 independent Laplace problems; each evaluation captured in a HIP graph, all graphs replayed back to
+    back (each context has its own streams, so small problems overlap)."""
+    from exafmm import abi
+    rng = np.random.default_rng(5)
+    ns = np.exp(rng.uniform(np.log(1e4), np.log(5e5), nprob)).astype(int)
+    t0 = time.perf_counter()
+    probs = [setup_generic(0, int(n), 8, 64, 0.0, "plummer", 1.0 / float(n), seed=100 + i) for i, n in enumerate(ns)]
+    t_setup = time.perf_counter() - t0
+    log("batched setup %.1fs, %d problems, %d particles" % (t_setup, nprob, int(ns.sum())))
+    res = {}
+    for pname, prec in (("mixed", abi.MIXED), ("fp64", abi.FP64)):
+        engs = [engine_for(pr, prec) for pr in probs]
+        for e in engs:
+            e.graph_capture()
+        for _ in range(max(1, a.warmup)):
+            for e in engs:
+                e.graph_launch()
+            for e in engs:
+                e.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            for e in engs:
+                e.graph_launch()
+            for e in engs:
+                e.synchronize()
+        dt = (time.perf_counter() - t0) / a.steps
+        errs = [sample_error_generic(pr, e.values(), 200, seed=7) for pr, e in zip(probs, engs)]
+        for e in engs:
+            e.close()
+        res[pname] = dict(ms=1e3 * dt, value=float(ns.sum()) / dt, err_phi=max(x[0] for x in errs),
+                          err_grad=max(x[1] for x in errs))
+        log("batched %s: %.3f ms per batch, max err phi %.2e grad %.2e" % (pname, 1e3 * dt, res[pname]["err_phi"],
+                                                                           res[pname]["err_grad"]))
+    m, f = res["mixed"], res["fp64"]
+    return {
+        "metric": "particles/s, laplace_batched FMM evaluation (aggregate)", "value": m["value"],
+        "unit": "particles/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": m["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 P2P/M2L + f64 pinv solves",
+        "data": "synthetic (seeded Plummer, scale radius 1, truncated at r=10; charges 1/N)",
+        "config": {"workload": "laplace_batched_%dproblems_p8_ncrit64" % nprob, "kernel": "laplace",
+                   "problems": nprob, "N_total": int(ns.sum()), "N_min": int(ns.min()), "N_max": int(ns.max()),
+                   "p": 8, "ncrit": 64, "graph": "one captured evaluation per problem"},
+        "err_phi": m["err_phi"], "err_grad": m["err_grad"], "fp64_value": f["value"], "fp64_ms_per_step": f["ms"],
+        "fp64_err_phi": f["err_phi"], "fp64_err_grad": f["err_grad"], "mixed_vs_fp64": m["value"] / f["value"],
+        "setup_s": round(t_setup, 2), "roofline": None, "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +319,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--mixed-only", action="store_true")
+    ap.add_argument("--workload", default="laplace_uniform",
+                    choices=["laplace_uniform", "laplace_batched"] + sorted(WORKLOADS))
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build evaluates one FMM on one GPU; multi-GPU runs are not part of it "
@@ -156,6 +330,13 @@ def main():
     if abi.device_count() < 1:
         print("bench.py: no GPU visible to the engine", file=sys.stderr)
         sys.exit(1)
+
+    if a.workload == "laplace_batched":
+        print(json.dumps(run_batched(a)), flush=True)
+        return
+    if a.workload != "laplace_uniform":
+        print(json.dumps(run_single_workload(a, a.workload)), flush=True)
+        return
 
     log("setup: N=%d p=%d ncrit=%d" % (a.n, a.p, a.ncrit))
     prob = setup_problem(a.n, a.p, a.ncrit, seed=0)

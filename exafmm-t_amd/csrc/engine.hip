@@ -86,6 +86,11 @@ struct xfmm_context {
   bool near_pending = false;   // `near` is being filled with P2P of the current charges
   bool near_launched = false;  // a P2P was ever queued on stream2 (ev[20] valid)
   bool near_overlapped = false;  // the last downward pass took P2P from stream2
+  // graph of one evaluation (xfmm_graph_capture); timing events are not
+  // recorded while capturing
+  bool capturing = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
   std::string err;
 
   // ---- operators
@@ -250,6 +255,17 @@ void require(bool c, const std::string& msg) {
   if (!c) throw ArgError{msg};
 }
 
+// A captured graph holds the device addresses of the buffers it was captured
+// with: drop it before anything is reallocated.
+void drop_graph(xfmm_context* c) {
+  if (!c->graph_exec && !c->graph) return;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->graph_exec) HIPCHK(hipGraphExecDestroy(c->graph_exec));
+  if (c->graph) HIPCHK(hipGraphDestroy(c->graph));
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+}
+
 // ---------------------------------------------------------------- operators
 
 void op_P2M(xfmm_context* c);
@@ -345,7 +361,7 @@ void op_P2P(xfmm_context* c) {
 }
 
 void record(xfmm_context* c, int i) {
-  if (c->ev_ok) HIPCHK(hipEventRecord(c->ev[i], c->stream));
+  if (c->ev_ok && !c->capturing) HIPCHK(hipEventRecord(c->ev[i], c->stream));
 }
 
 // Queue the near-field P2P of the current charges on the second stream, after
@@ -355,11 +371,11 @@ void launch_near(xfmm_context* c) {
   HIPCHK(hipEventRecord(c->ev[23], c->stream));
   HIPCHK(hipStreamWaitEvent(c->stream2, c->ev[23], 0));
   launch_zero(c->near.ptr, c->near.n, c->stream2);
-  HIPCHK(hipEventRecord(c->ev[21], c->stream2));
+  if (!c->capturing) HIPCHK(hipEventRecord(c->ev[21], c->stream2));
   DevTree t = c->dev_tree();
   t.trg_value = c->near.ptr;
   launch_eval_targets(c->kind, c->f32_p2p, MODE_P2P, t, c->wavek(), c->w_p2p.ptr, c->n_p2p, c->stream2);
-  HIPCHK(hipEventRecord(c->ev[22], c->stream2));
+  if (!c->capturing) HIPCHK(hipEventRecord(c->ev[22], c->stream2));
   HIPCHK(hipEventRecord(c->ev[20], c->stream2));
   c->near_pending = true;
   c->near_launched = true;
@@ -509,6 +525,8 @@ void xfmm_destroy(xfmm_context* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -531,6 +549,7 @@ int xfmm_set_operators(xfmm_context* c, int nlevels, const double* uc2e_u, const
                        const double* m2l) {
   return guarded(c, [&] {
     require(nlevels >= 1 && nm2l >= 1, "xfmm_set_operators: nlevels and nm2l must be >= 1");
+    drop_graph(c);
     require(uc2e_u && uc2e_v && dc2e_u && dc2e_v && m2m && l2l && m2l, "xfmm_set_operators: null operator");
     if (c->kind == XFMM_LAPLACE) require(nlevels == 1 && nm2l == 1, "xfmm_set_operators: Laplace operators are scale invariant (nlevels = nm2l = 1)");
     const int ns = c->nsurf, vw = c->vw;
@@ -647,6 +666,7 @@ int xfmm_set_tree(xfmm_context* c, const xfmm_tree* t) {
       }
 
     hipStream_t st = c->stream;
+    drop_graph(c);
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipStreamSynchronize(c->stream2));
     c->near_pending = false;
@@ -940,6 +960,43 @@ int xfmm_downward_pass(xfmm_context* c) {
     c->near_pending = false;
     c->near_overlapped = pend;
     HIPCHK(hipGetLastError());
+  });
+}
+
+int xfmm_graph_capture(xfmm_context* c) {
+  return guarded(c, [&] {
+    check_ready(c);
+    drop_graph(c);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream2));
+    c->near_pending = false;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    int rc = xfmm_upward_pass(c);
+    if (rc == XFMM_OK) rc = xfmm_downward_pass(c);
+    c->capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc != XFMM_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      throw ArgError{"xfmm_graph_capture: " + c->err};
+    }
+    HIPCHK(e);
+    c->graph = g;
+    HIPCHK(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+    c->near_pending = false;
+    c->down_clean = false;
+  });
+}
+
+int xfmm_graph_launch(xfmm_context* c) {
+  return guarded(c, [&] {
+    require(c->graph_exec != nullptr, "xfmm_graph_launch: no graph (call xfmm_graph_capture)");
+    // the graph zeroes and recomputes everything it produces; eager state
+    // afterwards is that of a completed evaluation
+    HIPCHK(hipGraphLaunch(c->graph_exec, c->stream));
+    c->near_pending = false;
+    c->down_clean = false;
   });
 }
 

@@ -60,6 +60,8 @@ def load(path=LIB_PATH):
             "xfmm_clear_values": ([vp], ctypes.c_int),
             "xfmm_upward_pass": ([vp], ctypes.c_int),
             "xfmm_downward_pass": ([vp], ctypes.c_int),
+            "xfmm_graph_capture": ([vp], ctypes.c_int),
+            "xfmm_graph_launch": ([vp], ctypes.c_int),
             "xfmm_get_values": ([vp, vp], ctypes.c_int),
             "xfmm_get_equiv": ([vp, vp, vp], ctypes.c_int),
             "xfmm_set_equiv": ([vp, vp, vp], ctypes.c_int),
@@ -204,6 +206,12 @@ class Engine:
 
     def downward_pass(self):
         self._check(self.lib.xfmm_downward_pass(self.h), "xfmm_downward_pass")
+
+    def graph_capture(self):
+        self._check(self.lib.xfmm_graph_capture(self.h), "xfmm_graph_capture")
+
+    def graph_launch(self):
+        self._check(self.lib.xfmm_graph_launch(self.h), "xfmm_graph_launch")
 
     def op(self, name):
         self._check(getattr(self.lib, "xfmm_" + name)(self.h), "xfmm_" + name)

@@ -133,6 +133,18 @@ int xfmm_clear_values(xfmm_context* ctx);
 int xfmm_upward_pass(xfmm_context* ctx);
 int xfmm_downward_pass(xfmm_context* ctx);
 
+/* One evaluation (upward_pass + downward_pass, the same kernels on the same
+ * two streams) captured into a HIP graph held by the context, and its replay
+ * on the context's stream.  A replay reads the charges set at the time it
+ * runs and leaves the same values as an eager evaluation.  For callers that
+ * evaluate many times, or many small problems (one graph per context, all
+ * replayed back to back); no counterpart in the reference, whose evaluation
+ * is host-driven.  The graph is recaptured by calling capture again, and
+ * dropped by xfmm_set_tree / xfmm_set_operators.  Per-operator timings are
+ * not recorded by replays. */
+int xfmm_graph_capture(xfmm_context* ctx);
+int xfmm_graph_launch(xfmm_context* ctx);
+
 /* Individual operators, same order and semantics as the reference
  * (accumulating into the device buffers; no zeroing):
  *   P2M  fmm_scale_invariant.h:163-195 / fmm.h:156-185

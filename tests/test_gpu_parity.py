@@ -186,3 +186,37 @@ def test_linearity_and_repeatability(require_gpu):
     eng.set_equiv(None, np.ones_like(eng.equiv()[1]))
     eng.downward_pass()
     assert np.array_equal(eng.values(), res[0])
+
+
+@pytest.mark.parametrize("kernel,precision", [("laplace", abi.MIXED), ("modified_helmholtz", abi.FP64),
+                                              ("helmholtz", abi.MIXED)])
+def test_graph_replay_matches_eager(require_gpu, kernel, precision):
+    """A captured evaluation replays to the eager values, replay after replay, with eager evaluations and new
+    charges in between (the graph holds kernels only: its clears are fill kernels)."""
+    pr = make_problem(kernel, 6000, 6, ncrit=32, dist="plummer", seed=21)
+    print("graph case", kernel, precision, flush=True)
+    eng = _engine(pr, precision)
+    rng = np.random.default_rng(8)
+    qs = [pr["qs"]] + [pr["qs"] * (rng.random(pr["qs"].size) + 0.5) for _ in range(3)]
+    eager = []
+    for q in qs:
+        eng.set_charges(q)
+        eng.upward_pass()
+        eng.downward_pass()
+        eager.append(eng.values())
+    eng.graph_capture()
+    for rep in range(2):
+        for i, q in enumerate(qs):
+            eng.set_charges(q)
+            eng.graph_launch()
+            eng.synchronize()
+            assert np.array_equal(eng.values(), eager[i]), (rep, i)
+            # eager work between replays
+            eng.set_charges(qs[(i + 1) % len(qs)])
+            eng.upward_pass()
+            eng.downward_pass()
+            assert np.array_equal(eng.values(), eager[(i + 1) % len(qs)]), (rep, i)
+    # a new tree drops the graph
+    eng.set_tree(pr["flat"])
+    with pytest.raises(abi.XfmmError):
+        eng.graph_launch()
