@@ -127,6 +127,7 @@ struct xfmm_context {
   DevBuf<char> fft_in, fft_out;
   // buffers
   DevBuf<double> up, dn, trg_value, check, tmp, gemm_scratch, near;
+  DevBuf<double> src_rel, src_pack;  // P2P sources relative to their leaf centre; packed (x, y, z, q)
 
   // ---- timing
   hipEvent_t ev[24] = {};
@@ -161,6 +162,9 @@ struct xfmm_context {
     t.p2l_off = p2l_off.ptr;
     t.p2l_src = p2l_src.ptr;
     t.unit_surf = unit_surf.ptr;
+    t.nsrcs = nsrcs;
+    t.src_rel = src_rel.ptr;
+    t.src_pack = src_pack.ptr;
     t.up = up.ptr;
     t.dn = dn.ptr;
     t.trg_value = trg_value.ptr;
@@ -642,7 +646,7 @@ int xfmm_set_tree(xfmm_context* c, const xfmm_tree* t) {
       const int sb = t->src_begin[i], sc = t->src_count[i], tb = t->trg_begin[i], tc = t->trg_count[i];
       require(sc >= 0 && tc >= 0 && sb >= 0 && tb >= 0 && sb + sc <= t->nsrcs && tb + tc <= t->ntrgs,
               "xfmm_set_tree: body range out of bounds");
-      require(leaf || (sc == 0 && tc == 0) || true, "");
+      require(leaf || sc == 0, "xfmm_set_tree: a non-leaf holds sources");
     }
     auto check_csr = [&](const int* off, const int* lst, const char* name, bool leaf_sources) {
       require(off[0] == 0, std::string("xfmm_set_tree: ") + name + " offsets must start at 0");
@@ -687,6 +691,16 @@ int xfmm_set_tree(xfmm_context* c, const xfmm_tree* t) {
     c->trg_begin.upload(t->trg_begin, nn, st);
     c->trg_count.upload(t->trg_count, nn, st);
     c->src.upload(t->src_coord, size_t(t->nsrcs) * 3, st);
+    {
+      std::vector<double> rel(size_t(t->nsrcs) * 3);
+      for (int i = 0; i < nn; i++) {
+        if (ch[i] >= 0) continue;
+        for (int k = t->src_begin[i]; k < t->src_begin[i] + t->src_count[i]; k++)
+          for (int d = 0; d < 3; d++) rel[3 * size_t(k) + d] = t->src_coord[3 * size_t(k) + d] - t->center[3 * i + d];
+      }
+      c->src_rel.upload(rel, st);
+      c->src_pack.alloc(std::max<size_t>(1, size_t(t->nsrcs) * 4));
+    }
     c->trg.upload(t->trg_coord, size_t(t->ntrgs) * 3, st);
     c->p2p_off.upload(t->p2p_offset, nn + 1, st);
     c->p2p_src.upload(t->p2p_source, t->p2p_offset[nn], st);

@@ -10,7 +10,7 @@ namespace xfmm {
 // kind T are doubles (real kernels) or interleaved (re, im) doubles
 // (Helmholtz); `vw` is the number of doubles per value (1 or 2).
 struct DevTree {
-  int nnodes = 0, depth = 0, p = 0, nsurf = 0, vw = 1;
+  int nnodes = 0, depth = 0, p = 0, nsurf = 0, vw = 1, nsrcs = 0;
   double r0 = 0;
   const double* center = nullptr;  // [nnodes][3]
   const int* level = nullptr;      // [nnodes]
@@ -29,6 +29,8 @@ struct DevTree {
   const int* p2l_off = nullptr;
   const int* p2l_src = nullptr;
   const double* unit_surf = nullptr;  // [nsurf][3] unit-cube surface (reference point order)
+  const double* src_rel = nullptr;    // [nsrcs][3] source positions relative to their leaf centre
+  void* src_pack = nullptr;           // [nsrcs] (x, y, z, q) relative to the leaf centre, in the P2P type
   double* up = nullptr;               // T[nnodes][nsurf]
   double* dn = nullptr;               // T[nnodes][nsurf]
   double* trg_value = nullptr;        // T[ntrgs][4]

@@ -139,6 +139,114 @@ __device__ __forceinline__ void tile_sum(int m, const R* sx, const R* sy, const 
   }
 }
 
+// ------------------------------------------------------------------ P2P
+//
+// Near field of the real kernels with the sources read as wave-uniform
+// 16/32-byte vectors (scalar loads, no LDS and no barriers): each source is
+// packed once per evaluation as (x, y, z, q) relative to its leaf centre; a
+// target is moved into the frame of the source leaf with one subtraction per
+// source leaf (the offset of the two centres is formed in FP64).  Pairs are
+// summed per source leaf in chunks of 64 (plain sums, FP32 chunks combined
+// with Kahan summation, as in the tiled kernel).
+template <typename R>
+struct P4;
+template <>
+struct P4<float> {
+  typedef float4 t;
+};
+template <>
+struct P4<double> {
+  typedef double4 t;
+};
+
+template <typename R>
+__global__ void k_pack_sources(int n, const double* __restrict__ rel, const double* __restrict__ q,
+                               typename P4<R>::t* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  typename P4<R>::t v;
+  v.x = R(rel[3 * size_t(i)]);
+  v.y = R(rel[3 * size_t(i) + 1]);
+  v.z = R(rel[3 * size_t(i) + 2]);
+  v.w = R(q[i]);
+  out[i] = v;
+}
+
+template <int KIND, typename R>
+__global__ __launch_bounds__(64) void k_p2p_smem(DevTree t, Wavek wk, const int* __restrict__ work) {
+  constexpr int NA = 4;
+  // constant address space: the packed sources are read-only for the whole
+  // launch, and wave-uniform loads from it become scalar loads
+  typedef const R __attribute__((address_space(4))) * CR;
+  const CR src = (CR)(t.src_pack);
+  const int leaf = work[blockIdx.x];
+  const int lane = threadIdx.x;
+  const double cx = t.center[3 * leaf], cy = t.center[3 * leaf + 1], cz = t.center[3 * leaf + 2];
+  const int tb = t.trg_begin[leaf], nt = t.trg_count[leaf];
+  const R kr = R(wk.re), ki = R(wk.im);
+  const int g0 = t.p2p_off[leaf], g1 = t.p2p_off[leaf + 1];
+  for (int t0 = 0; t0 < nt; t0 += TILE) {
+    const int ti = t0 + lane;
+    const bool act = ti < nt;
+    R tx = 0, ty = 0, tz = 0;
+    if (act) {
+      const double* x = t.trg + 3 * size_t(tb + ti);
+      tx = R(x[0] - cx);
+      ty = R(x[1] - cy);
+      tz = R(x[2] - cz);
+    }
+    R acc[NA], comp[NA];
+#pragma unroll
+    for (int a = 0; a < NA; a++) { acc[a] = 0; comp[a] = 0; }
+    for (int g = g0; g < g1; g++) {
+      const int snode = t.p2p_src[g];
+      const int sb = t.src_begin[snode], ns = t.src_count[snode];
+      // target in the frame of the source leaf
+      const R ux = tx - R(t.center[3 * snode] - cx);
+      const R uy = ty - R(t.center[3 * snode + 1] - cy);
+      const R uz = tz - R(t.center[3 * snode + 2] - cz);
+      const bool self = snode == leaf;
+      for (int s0 = 0; s0 < ns; s0 += TILE) {
+        const int m = min(TILE, ns - s0);
+        const CR S = src + 4 * size_t(sb + s0);
+        R part[NA] = {0, 0, 0, 0};
+        int j = 0;
+        if (self) {
+          for (; j + 4 <= m; j += 4) {
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const CR v = S + 4 * (j + u);
+              pair_grad<KIND, R, RSQ_MASK>(v[0] - ux, v[1] - uy, v[2] - uz, v[3], R(0), kr, ki, part);
+            }
+          }
+          for (; j < m; j++) {
+            const CR v = S + 4 * j;
+            pair_grad<KIND, R, RSQ_MASK>(v[0] - ux, v[1] - uy, v[2] - uz, v[3], R(0), kr, ki, part);
+          }
+        } else {
+          for (; j + 4 <= m; j += 4) {
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const CR v = S + 4 * (j + u);
+              pair_grad<KIND, R, RSQ_SEP>(v[0] - ux, v[1] - uy, v[2] - uz, v[3], R(0), kr, ki, part);
+            }
+          }
+          for (; j < m; j++) {
+            const CR v = S + 4 * j;
+            pair_grad<KIND, R, RSQ_SEP>(v[0] - ux, v[1] - uy, v[2] - uz, v[3], R(0), kr, ki, part);
+          }
+        }
+        fold_tile<R, NA>(acc, comp, part);
+      }
+    }
+    if (act) {
+      double* o = t.trg_value + size_t(tb + ti) * 4;
+#pragma unroll
+      for (int a = 0; a < NA; a++) o[a] += (double(acc[a]) - double(comp[a])) * FOUR_PI_INV;
+    }
+  }
+}
+
 template <int KIND, typename R, int MODE>
 __global__ __launch_bounds__(64) void k_eval_targets(DevTree t, Wavek wk, const int* __restrict__ work) {
   constexpr bool C = kcplx<KIND>();
@@ -356,7 +464,12 @@ __global__ void k_direct_reduce(const double* __restrict__ partial, int nsplit, 
 template <int KIND, typename R>
 static void eval_targets_t(int mode, const DevTree& t, Wavek k, const int* work, int nwork, hipStream_t s) {
   if (nwork <= 0) return;
-  if (mode == MODE_P2P) hipLaunchKernelGGL((k_eval_targets<KIND, R, MODE_P2P>), dim3(nwork), dim3(64), 0, s, t, k, work);
+  if (mode == MODE_P2P && KIND != HELMHOLTZ && t.src_pack && t.src_rel) {
+    if (t.nsrcs > 0)
+      hipLaunchKernelGGL(k_pack_sources<R>, dim3((t.nsrcs + 255) / 256), dim3(256), 0, s, t.nsrcs, t.src_rel, t.q,
+                         static_cast<typename P4<R>::t*>(t.src_pack));
+    hipLaunchKernelGGL((k_p2p_smem<KIND, R>), dim3(nwork), dim3(64), 0, s, t, k, work);
+  } else if (mode == MODE_P2P) hipLaunchKernelGGL((k_eval_targets<KIND, R, MODE_P2P>), dim3(nwork), dim3(64), 0, s, t, k, work);
   else if (mode == MODE_M2P) hipLaunchKernelGGL((k_eval_targets<KIND, R, MODE_M2P>), dim3(nwork), dim3(64), 0, s, t, k, work);
   else hipLaunchKernelGGL((k_eval_targets<KIND, R, MODE_L2P>), dim3(nwork), dim3(64), 0, s, t, k, work);
 }
