@@ -101,7 +101,6 @@ struct xfmm_context {
   DevBuf<float> m2l32;
   DevBuf<double> unit_surf;
   DevBuf<int> map_up, map_dn;
-  DevBuf<int> up_line_off, up_line_ent;  // input surface points grouped by x line
 
   // ---- tree
   bool have_tree = false;
@@ -181,8 +180,6 @@ struct xfmm_context {
     m.mixed_fft = f32_m2l_fft;
     m.map_up = map_up.ptr;
     m.map_dn = map_dn.ptr;
-    m.up_line_off = up_line_off.ptr;
-    m.up_line_ent = up_line_ent.ptr;
     m.fft_in = fft_in.ptr;
     m.fft_out = fft_out.ptr;
     return m;
@@ -492,19 +489,6 @@ int xfmm_create(int kernel, int p, double wavek_re, double wavek_im, int precisi
         seen[v] = 1;
       }
       (up ? c->map_up : c->map_dn).upload(m, c->stream);
-      if (up) {
-        // CSR of the input points by x line (y, z): line y p + z, entry (i << 4) | x
-        std::vector<int> off(size_t(p) * p + 1, 0), ent(m.size());
-        for (int v : m) off[(v / p % p) * p + v / (p * p) + 1]++;
-        for (size_t l = 0; l + 1 < off.size(); l++) off[l + 1] += off[l];
-        std::vector<int> pos(off.begin(), off.end() - 1);
-        for (size_t i = 0; i < m.size(); i++) {
-          const int v = m[i];
-          ent[pos[(v / p % p) * p + v / (p * p)]++] = int(i << 4) | (v % p);
-        }
-        c->up_line_off.upload(off, c->stream);
-        c->up_line_ent.upload(ent, c->stream);
-      }
     }
     M2LDev m = c->m2l_dev();
     const size_t lds = m2l_lds_bytes(m);

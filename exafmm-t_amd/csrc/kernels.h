@@ -77,8 +77,6 @@ struct M2LDev {
   bool mixed_fft = false;            // FP32 transform arithmetic too (else FP64 arithmetic, FP32 storage)
   const int* map_up = nullptr;       // [nsurf] local index x + p (y + p z) in the [0,p)^3 input cube
   const int* map_dn = nullptr;       // [nsurf] local index in the [p,2p)^3 output cube (offset p removed)
-  const int* up_line_off = nullptr;  // [p*p + 1] CSR of input surface points by x line, line = y p + z
-  const int* up_line_ent = nullptr;  // [nsurf] entry = (surface index << 4) | x
   void* fft_in = nullptr;            // [nsrc_slots][nfreq][8] complex
   void* fft_out = nullptr;           // [ntrg_slots][nfreq][8] complex
 };
