@@ -100,7 +100,7 @@ struct xfmm_context {
   DevBuf<double> m2l64;  // M2L operators in the expanded MFMA form, [L][nfreq][26][64][4]
   DevBuf<float> m2l32;
   DevBuf<double> unit_surf;
-  DevBuf<int> map_up, map_dn;
+  DevBuf<int> map_up, map_dn, inv_dn;
 
   // ---- tree
   bool have_tree = false;
@@ -180,6 +180,7 @@ struct xfmm_context {
     m.mixed_fft = f32_m2l_fft;
     m.map_up = map_up.ptr;
     m.map_dn = map_dn.ptr;
+    m.inv_dn = inv_dn.ptr;
     m.fft_in = fft_in.ptr;
     m.fft_out = fft_out.ptr;
     return m;
@@ -489,6 +490,11 @@ int xfmm_create(int kernel, int p, double wavek_re, double wavek_im, int precisi
         seen[v] = 1;
       }
       (up ? c->map_up : c->map_dn).upload(m, c->stream);
+      if (!up) {
+        std::vector<int> inv(size_t(p) * p * p, -1);
+        for (size_t i = 0; i < m.size(); i++) inv[m[i]] = int(i);
+        c->inv_dn.upload(inv, c->stream);
+      }
     }
     M2LDev m = c->m2l_dev();
     const size_t lds = m2l_lds_bytes(m);
@@ -583,14 +589,14 @@ int xfmm_set_operators(xfmm_context* c, int nlevels, const double* uc2e_u, const
       DevBuf<float> compact;
       compact.upload(f, c->stream);
       c->m2l32.alloc(2 * msz);
-      launch_m2l_expand(true, nm2l, c->nfreq, compact.ptr, c->m2l32.ptr, c->stream);
+      launch_m2l_expand(c->m2l_dev(), nm2l, compact.ptr, c->m2l32.ptr, c->stream);
       HIPCHK(hipStreamSynchronize(c->stream));
       c->m2l64.release();
     } else {
       DevBuf<double> compact;
       compact.upload(m2l, msz, c->stream);
       c->m2l64.alloc(2 * msz);
-      launch_m2l_expand(false, nm2l, c->nfreq, compact.ptr, c->m2l64.ptr, c->stream);
+      launch_m2l_expand(c->m2l_dev(), nm2l, compact.ptr, c->m2l64.ptr, c->stream);
       HIPCHK(hipStreamSynchronize(c->stream));
       c->m2l32.release();
     }

@@ -77,6 +77,7 @@ struct M2LDev {
   bool mixed_fft = false;            // FP32 transform arithmetic too (else FP64 arithmetic, FP32 storage)
   const int* map_up = nullptr;       // [nsurf] local index x + p (y + p z) in the [0,p)^3 input cube
   const int* map_dn = nullptr;       // [nsurf] local index in the [p,2p)^3 output cube (offset p removed)
+  const int* inv_dn = nullptr;       // [p^3] inverse of map_dn: surface index of a cube point, or -1
   void* fft_in = nullptr;            // [nsrc_slots][nfreq][8] complex
   void* fft_out = nullptr;           // [ntrg_slots][nfreq][8] complex
 };
@@ -102,8 +103,9 @@ struct HadBlocks {
 void launch_hadamard(const M2LDev& m, const void* m2l_opsx, const HadBlocks& hb, hipStream_t s);
 // Expand nlev operator levels from the reference layout [L][26][nfreq][64]
 // complex into the per-lane MFMA A-operand form [L][nfreq][26][64 lanes][4]
-// (float if f32, else double).
-void launch_m2l_expand(bool f32, int nlev, int nfreq, const void* M, void* X, hipStream_t s);
+// (float for the mixed path, else double), frequencies in the order of the
+// transforms that m selects.
+void launch_m2l_expand(const M2LDev& m, int nlev, const void* M, void* X, hipStream_t s);
 // ifft_dn_check: one (target slot, child) per work item, adds
 // scale[slot] * result to the child's dn vector.
 void launch_ifft_dn(const M2LDev& m, const DevTree& t, const int* trg_node, const double* scale, int nslots,

@@ -286,9 +286,12 @@ struct Quad<double> {
 //   j >= 8: { Im M[g][t],  Im M[g+4][t],  Re M[g][t],  Re M[g+4][t] },  t = j & 7,
 // stored as X[((L nfreq + k) 26 + r) 64 + 16 g + j], so a wave reads one
 // contiguous 1 KiB per (k, r).  Built once from the reference layout
-// M[L][r][k][8 s + t] (complex).
+// M[L][r][f][8 s + t] (complex, f = kx + (p+1) (ky + 2p kz)).  With
+// slab = true, X (and the spectra of the per-node transforms) number the
+// frequencies kx-slab first, k = kx (2p)^2 + ky + 2p kz, so that one kx slab
+// of a node's spectrum is contiguous in memory.
 template <typename R>
-__global__ void k_m2l_expand(int nlev, int nfreq, const R* __restrict__ M, R* __restrict__ X) {
+__global__ void k_m2l_expand(int nlev, int nfreq, int n1, bool slab, const R* __restrict__ M, R* __restrict__ X) {
   const size_t n = size_t(nlev) * nfreq * NM2L * 64;
   for (size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += size_t(gridDim.x) * blockDim.x) {
     const int lane = int(e % 64), g = lane >> 4, j = lane & 15, t = j & 7;
@@ -296,7 +299,12 @@ __global__ void k_m2l_expand(int nlev, int nfreq, const R* __restrict__ M, R* __
     const int r = int(q % NM2L);
     const size_t lk = q / NM2L;  // L * nfreq + k
     const int k = int(lk % nfreq), L = int(lk / nfreq);
-    const R* src = M + ((size_t(L) * NM2L + r) * nfreq + k) * 128;
+    int f = k;
+    if (slab) {
+      const int kx = k / (n1 * n1), rem = k % (n1 * n1);
+      f = kx + (n1 / 2 + 1) * rem;  // rem = ky + n1 kz
+    }
+    const R* src = M + ((size_t(L) * NM2L + r) * nfreq + f) * 128;
     const R m0x = src[2 * (g * 8 + t)], m0y = src[2 * (g * 8 + t) + 1];
     const R m1x = src[2 * ((g + 4) * 8 + t)], m1y = src[2 * ((g + 4) * 8 + t) + 1];
     R* o = X + e * 4;
@@ -305,11 +313,16 @@ __global__ void k_m2l_expand(int nlev, int nfreq, const R* __restrict__ M, R* __
   }
 }
 
-void launch_m2l_expand(bool f32, int nlev, int nfreq, const void* M, void* X, hipStream_t s) {
-  const size_t n = size_t(nlev) * nfreq * NM2L * 64;
+void launch_m2l_expand(const M2LDev& m, int nlev, const void* M, void* X, hipStream_t s) {
+  const size_t n = size_t(nlev) * m.nfreq * NM2L * 64;
   const unsigned blocks = unsigned(std::min<size_t>((n + 255) / 256, 65536));
-  if (f32) hipLaunchKernelGGL(k_m2l_expand<float>, dim3(blocks), dim3(256), 0, s, nlev, nfreq, static_cast<const float*>(M), static_cast<float*>(X));
-  else hipLaunchKernelGGL(k_m2l_expand<double>, dim3(blocks), dim3(256), 0, s, nlev, nfreq, static_cast<const double*>(M), static_cast<double*>(X));
+  const bool slab = fft_node_ok(m);
+  if (m.mixed)
+    hipLaunchKernelGGL(k_m2l_expand<float>, dim3(blocks), dim3(256), 0, s, nlev, m.nfreq, m.n1, slab,
+                       static_cast<const float*>(M), static_cast<float*>(X));
+  else
+    hipLaunchKernelGGL(k_m2l_expand<double>, dim3(blocks), dim3(256), 0, s, nlev, m.nfreq, m.n1, slab,
+                       static_cast<const double*>(M), static_cast<double*>(X));
 }
 
 // Per block of <= 64 target slots and a chunk of HAD_KCH frequencies.  For
@@ -575,7 +588,7 @@ __global__ __launch_bounds__(256) void k_fft_up_mfma(M2LDev m, DevTree t, const 
 #pragma unroll
         for (int h = 0; h < 2; h++) {
           const int kz = mt * 8 + RowMap<R>::fd(g, 2 * h);
-          if (kz < n1) out[size_t(kx + nh * (ky + n1 * kz)) * NCHILD + c] = C2<RS>{RS(acc[2 * h]), RS(acc[2 * h + 1])};
+          if (kz < n1) out[size_t(kx * n1 * n1 + ky + n1 * kz) * NCHILD + c] = C2<RS>{RS(acc[2 * h]), RS(acc[2 * h + 1])};
         }
       }
     }
@@ -599,8 +612,7 @@ __global__ __launch_bounds__(256) void k_ifft_dn_mfma(M2LDev m, DevTree t, const
   constexpr int NCZ = 8 * n1, NTZ = NCZ / 16;        // z pass columns (ky, c)
   constexpr int NTZW = (NTZ + 3) / 4;                // z tiles per wave (at most)
   constexpr int NCY = 8 * P, NTY = (NCY + 15) / 16;  // y pass columns (z, c)
-  constexpr int NSMAX = 6 * (P - 1) * (P - 1) + 2;   // surface points
-  constexpr int NG = (NCHILD * NSMAX + 255) / 256;   // surface values per thread
+  constexpr int NL = (NCHILD * P * P + 255) / 256;   // x lines per thread
   extern __shared__ char smem_raw[];
   const int slot = blockIdx.x;
   if (slot >= nslots) return;
@@ -619,23 +631,14 @@ __global__ __launch_bounds__(256) void k_ifft_dn_mfma(M2LDev m, DevTree t, const
       const int f = mt * 8 + RowMap<R>::fa(j), k = ks * 4 + g, v = k >> 1;
       wf[mt][ks] = (f < P) ? R(dft_entry(v * (P + f), n1, +1, RowMap<R>::ra(j), k & 1)) : R(0);
     }
-  // surface results of this thread, element e = tid + 256 r = (c, i), and
-  // packed per element: bits 0-11 the y-pass output it reads, 12-16 its x
-  // step p + x, 17-21 the running phase kx (p + x) mod n1 (-1: no element)
-  static_assert(P * P * NCHILD < 4096 && 2 * P < 32, "packing");
-  R gacc[NG];
-  int st[NG];
+  // x (c2r) results of this thread: the whole x line of y-pass outputs
+  // e = tid + 256 r = (z, y, c) (Ap index e), i.e. the grid points x in
+  // [p, 2p) of that line, accumulated over the kx slabs in registers
+  R gx[NL][P];
 #pragma unroll
-  for (int r = 0; r < NG; r++) {
-    const int e = tid + 256 * r;
-    gacc[r] = R(0);
-    st[r] = -1;
-    if (e < NCHILD * ns) {
-      const int c = e / ns, i = e - c * ns, gi = m.map_dn[i];
-      const int x = gi % P, y = (gi / P) % P, z = gi / (P * P);
-      st[r] = ((z * P + y) * NCHILD + c) | ((P + x) << 12);
-    }
-  }
+  for (int r = 0; r < NL; r++)
+#pragma unroll
+    for (int x = 0; x < P; x++) gx[r][x] = R(0);
   const C2<RS>* in = reinterpret_cast<const C2<RS>*>(m.fft_out) + size_t(slot) * m.nfreq * NCHILD;
   // z-pass input fragments of this wave's tiles, loaded one slab ahead
   RS bz[NTZW][KS];
@@ -649,7 +652,7 @@ __global__ __launch_bounds__(256) void k_ifft_dn_mfma(M2LDev m, DevTree t, const
         const int k = ks * 4 + g;
         bz[s][ks] = RS(0);
         if (nt < NTZ) {
-          const RS* src = reinterpret_cast<const RS*>(in + size_t(kx + nh * (ky + n1 * (k >> 1))) * NCHILD + c);
+          const RS* src = reinterpret_cast<const RS*>(in + size_t(kx * n1 * n1 + ky + n1 * (k >> 1)) * NCHILD + c);
           bz[s][ks] = src[k & 1];
         }
       }
@@ -701,20 +704,22 @@ __global__ __launch_bounds__(256) void k_ifft_dn_mfma(M2LDev m, DevTree t, const
       }
     }
     __syncthreads();
-    // x (c2r) at the check-surface points, weight 1 for kx = 0 and the
-    // Nyquist plane, else 2
+    // x (c2r) over each line, weight 1 for kx = 0 and the Nyquist plane,
+    // else 2 (the twiddles are wave-uniform)
     const R wt = (kx == 0 || kx == nh - 1) ? R(1) : R(2);
+    C2<R> tw[P];
 #pragma unroll
-    for (int r = 0; r < NG; r++) {
-      if (st[r] >= 0) {
-        const C2<R> a = Ap[st[r] & 4095];
-        int ph = st[r] >> 17;
-        const C2<R> tw = w[ph];
-        gacc[r] += wt * (a.x * tw.x - a.y * tw.y);
-        ph += (st[r] >> 12) & 31;
-        ph -= (ph >= n1) ? n1 : 0;
-        ph -= (ph >= n1) ? n1 : 0;
-        st[r] = (st[r] & 131071) | (ph << 17);
+    for (int x = 0; x < P; x++) {
+      const C2<R> v = w[(kx * (P + x)) % n1];
+      tw[x] = C2<R>{wt * v.x, wt * v.y};
+    }
+#pragma unroll
+    for (int r = 0; r < NL; r++) {
+      const int e = tid + 256 * r;
+      if (e < NCHILD * P * P) {
+        const C2<R> a = Ap[e];
+#pragma unroll
+        for (int x = 0; x < P; x++) gx[r][x] += a.x * tw[x].x - a.y * tw[x].y;
       }
     }
     // the next z pass writes Bp only (its readers are behind the barrier
@@ -723,9 +728,19 @@ __global__ __launch_bounds__(256) void k_ifft_dn_mfma(M2LDev m, DevTree t, const
   const int node = trg_node[slot];
   const double sc = scale[slot];
   double* dn = t.dn + size_t(t.child0[node]) * ns;
+  // scatter the surface points of the lines (each surface point is one
+  // grid point of one line)
 #pragma unroll
-  for (int r = 0; r < NG; r++)
-    if (st[r] >= 0) dn[tid + 256 * r] += sc * double(gacc[r]);
+  for (int r = 0; r < NL; r++) {
+    const int e = tid + 256 * r;
+    if (e >= NCHILD * P * P) continue;
+    const int c = e & 7, zy = e >> 3;
+#pragma unroll
+    for (int x = 0; x < P; x++) {
+      const int i = m.inv_dn[zy * P + x];
+      if (i >= 0) dn[size_t(c) * ns + i] += sc * double(gx[r][x]);
+    }
+  }
 }
 
 // --------------------------------------------------------------- launchers
