@@ -607,7 +607,8 @@ __global__ __launch_bounds__(256) void k_ifft_dn_mfma(M2LDev m, DevTree t, const
                                                       const double* __restrict__ scale, int nslots) {
   typedef typename MfmaT<R>::v4 v4;
   constexpr int n1 = 2 * P, nh = P + 1;
-  constexpr int KK = 2 * n1, KS = KK / 4;            // inner: (kz or ky, re/im)
+  constexpr int KK = 2 * n1;                         // inner: (kz or ky, re/im)
+  constexpr int KS = 2 * ((n1 + 3) / 4);             // k-steps (pairs of steps share a frequency)
   constexpr int MT = (2 * P + 15) / 16;              // rows: (z or y in [p,2p), re/im)
   constexpr int NCZ = 8 * n1, NTZ = NCZ / 16;        // z pass columns (ky, c)
   constexpr int NTZW = (NTZ + 3) / 4;                // z tiles per wave (at most)
@@ -623,13 +624,15 @@ __global__ __launch_bounds__(256) void k_ifft_dn_mfma(M2LDev m, DevTree t, const
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int j = lane & 15, g = lane >> 4;
   make_twiddles(w, n1, +1);
+  // inner index of k-step ks, lane group g: frequency (ks / 2) 4 + g, re/im
+  // ks % 2, so that one complex input feeds a lane's two consecutive k-steps
   R wf[MT][KS];
 #pragma unroll
   for (int mt = 0; mt < MT; mt++)
 #pragma unroll
     for (int ks = 0; ks < KS; ks++) {
-      const int f = mt * 8 + RowMap<R>::fa(j), k = ks * 4 + g, v = k >> 1;
-      wf[mt][ks] = (f < P) ? R(dft_entry(v * (P + f), n1, +1, RowMap<R>::ra(j), k & 1)) : R(0);
+      const int f = mt * 8 + RowMap<R>::fa(j), v = (ks >> 1) * 4 + g;
+      wf[mt][ks] = (f < P && v < n1) ? R(dft_entry(v * (P + f), n1, +1, RowMap<R>::ra(j), ks & 1)) : R(0);
     }
   // x (c2r) results of this thread: the whole x line of y-pass outputs
   // e = tid + 256 r = (z, y, c) (Ap index e), i.e. the grid points x in
@@ -648,13 +651,11 @@ __global__ __launch_bounds__(256) void k_ifft_dn_mfma(M2LDev m, DevTree t, const
       const int nt = wave + 4 * s;
       const int col = nt * 16 + j, ky = col >> 3, c = col & 7;
 #pragma unroll
-      for (int ks = 0; ks < KS; ks++) {
-        const int k = ks * 4 + g;
-        bz[s][ks] = RS(0);
-        if (nt < NTZ) {
-          const RS* src = reinterpret_cast<const RS*>(in + size_t(kx * n1 * n1 + ky + n1 * (k >> 1)) * NCHILD + c);
-          bz[s][ks] = src[k & 1];
-        }
+      for (int kp = 0; kp < KS / 2; kp++) {
+        C2<RS> v{RS(0), RS(0)};
+        if (nt < NTZ && kp * 4 + g < n1) v = in[size_t(kx * n1 * n1 + ky + n1 * (kp * 4 + g)) * NCHILD + c];
+        bz[s][2 * kp] = v.x;
+        bz[s][2 * kp + 1] = v.y;
       }
     }
   };
@@ -689,7 +690,10 @@ __global__ __launch_bounds__(256) void k_ifft_dn_mfma(M2LDev m, DevTree t, const
       const int col = nt * 16 + j;
       R b[KS];
 #pragma unroll
-      for (int ks = 0; ks < KS; ks++) b[ks] = (col < NCY) ? Bp[(ks * 4 + g) * NCY + col] : R(0);
+      for (int ks = 0; ks < KS; ks++) {
+        const int v = (ks >> 1) * 4 + g;
+        b[ks] = (col < NCY && v < n1) ? Bp[(2 * v + (ks & 1)) * NCY + col] : R(0);
+      }
       const int z = col >> 3, c = col & 7;
 #pragma unroll
       for (int mt = 0; mt < MT; mt++) {
