@@ -70,6 +70,7 @@ struct xfmm_context {
   // stages that run in FP32 (mixed path only): near-field P2P, M2L
   // (transforms + Hadamard), and the surface evaluations P2M/L2P/M2P/P2L
   bool f32_p2p = false, f32_m2l = false, f32_surf = false, f32_m2l_fft = false, f32_far = false;
+  bool f16_had = false;  // split-FP16 Hadamard (needs f32_m2l)
   // dn and trg_value are zero (cleared by the last upward pass or clear_values
   // and not written since), so the downward pass need not clear them itself
   bool down_clean = false;
@@ -98,7 +99,9 @@ struct xfmm_context {
   int nlevels = 0, nm2l = 0;
   DevBuf<double> uc2e_uT, uc2e_vT, dc2e_uT, dc2e_vT, m2mcat, l2lcat;
   DevBuf<double> m2l64;  // M2L operators in the expanded MFMA form, [L][nfreq][26][64][4]
-  DevBuf<float> m2l32;
+  DevBuf<float> m2l32;   // FP32 form, or the split-FP16 form of the same size when f16_had
+  DevBuf<float> m2l_hscale;  // f16_had: operator scale per [L][nfreq]
+  DevBuf<double> m2l_amax;   // f16_had: largest |spectrum value| of the evaluation (float bits)
   DevBuf<double> unit_surf;
   DevBuf<int> map_up, map_dn, inv_dn;
 
@@ -183,6 +186,7 @@ struct xfmm_context {
     m.inv_dn = inv_dn.ptr;
     m.fft_in = fft_in.ptr;
     m.fft_out = fft_out.ptr;
+    m.amax = f16_had ? reinterpret_cast<unsigned*>(m2l_amax.ptr) : nullptr;
     return m;
   }
   Wavek wavek() const { return Wavek{wk_re, wk_im}; }
@@ -388,6 +392,7 @@ void op_M2L(xfmm_context* c, bool timed) {
   M2LDev m = c->m2l_dev();
   DevTree t = c->dev_tree();
   if (timed) record(c, 16);
+  if (m.amax) launch_zero(c->m2l_amax.ptr, 1, c->stream);
   launch_fft_up(m, t, c->m2l_src_node.ptr, c->n_m2l_src, c->stream);
   if (timed) record(c, 17);
   const size_t opsz = size_t(NM2L) * c->nfreq * 64 * 4;
@@ -406,7 +411,7 @@ void op_M2L(xfmm_context* c, bool timed) {
     hb.uni = c->m2l_blk_uni.ptr + size_t(b) * HAD_UCAP;
     hb.nunion = c->m2l_blk_nu.ptr + b;
     hb.loc = c->m2l_blk_loc.ptr + size_t(b) * NM2L * 64;
-    launch_hadamard(m, ops, hb, c->stream);
+    launch_hadamard(m, ops, c->f16_had ? c->m2l_hscale.ptr + size_t(L) * c->nfreq : nullptr, hb, c->stream);
   }
   if (timed) record(c, 18);
   launch_ifft_dn(m, t, c->m2l_trg_node.ptr, c->m2l_scale.ptr, c->n_m2l_trg, c->stream);
@@ -443,21 +448,24 @@ int xfmm_create(int kernel, int p, double wavek_re, double wavek_im, int precisi
   c->prec = precision;
   if (precision == XFMM_MIXED) {
     // default mixed policy: FP32 near field and FP32 M2L (spectra, Hadamard
-    // and transform arithmetic).  XFMM_MIXED_STAGES overrides it with a comma
-    // list of p2p, m2l (FP32 spectra + Hadamard), fft (FP32 transform
-    // arithmetic, else FP64), far (FP32 L2P / M2P evaluation) and surf (FP32
-    // check potentials of P2M / P2L).  "far" is off by default: the FP32 M2L
+    // and transform arithmetic), the Hadamard on split FP16 (two FP16 numbers
+    // per FP32 value, FP32 accumulation).  XFMM_MIXED_STAGES overrides it with
+    // a comma list of p2p, m2l (FP32 spectra + Hadamard), had16 (split-FP16
+    // Hadamard, with m2l), fft (FP32 transform arithmetic, else FP64), far
+    // (FP32 L2P / M2P evaluation) and surf (FP32 check potentials of P2M /
+    // P2L).  "far" is off by default: the FP32 M2L
     // noise, amplified by the downward pseudo-inverse, gives the local
     // equivalent densities large components that cancel only when the L2P
     // sum is done in FP64.  "surf" loses all accuracy through the
     // pseudo-inverses; both are kept for study.
     const char* env = std::getenv("XFMM_MIXED_STAGES");
-    const std::string st = env ? env : "p2p,m2l,fft";
+    const std::string st = env ? env : "p2p,m2l,had16,fft";
     c->f32_p2p = st.find("p2p") != std::string::npos;
     c->f32_m2l = st.find("m2l") != std::string::npos;
     c->f32_surf = st.find("surf") != std::string::npos;
     c->f32_far = st.find("far") != std::string::npos;
     c->f32_m2l_fft = c->f32_m2l && st.find("fft") != std::string::npos;
+    c->f16_had = c->f32_m2l && st.find("had16") != std::string::npos;
   }
   c->device = device;
   c->wk_re = wavek_re;
@@ -589,14 +597,19 @@ int xfmm_set_operators(xfmm_context* c, int nlevels, const double* uc2e_u, const
       DevBuf<float> compact;
       compact.upload(f, c->stream);
       c->m2l32.alloc(2 * msz);
-      launch_m2l_expand(c->m2l_dev(), nm2l, compact.ptr, c->m2l32.ptr, c->stream);
+      if (c->f16_had) {
+        c->m2l_hscale.alloc(size_t(nm2l) * c->nfreq);
+        if (!c->m2l_amax.ptr) c->m2l_amax.alloc(1);
+      }
+      launch_m2l_expand(c->m2l_dev(), nm2l, compact.ptr, c->m2l32.ptr, c->f16_had ? c->m2l_hscale.ptr : nullptr,
+                        c->stream);
       HIPCHK(hipStreamSynchronize(c->stream));
       c->m2l64.release();
     } else {
       DevBuf<double> compact;
       compact.upload(m2l, msz, c->stream);
       c->m2l64.alloc(2 * msz);
-      launch_m2l_expand(c->m2l_dev(), nm2l, compact.ptr, c->m2l64.ptr, c->stream);
+      launch_m2l_expand(c->m2l_dev(), nm2l, compact.ptr, c->m2l64.ptr, nullptr, c->stream);
       HIPCHK(hipStreamSynchronize(c->stream));
       c->m2l32.release();
     }

@@ -80,6 +80,7 @@ struct M2LDev {
   const int* inv_dn = nullptr;       // [p^3] inverse of map_dn: surface index of a cube point, or -1
   void* fft_in = nullptr;            // [nsrc_slots][nfreq][8] complex
   void* fft_out = nullptr;           // [ntrg_slots][nfreq][8] complex
+  unsigned* amax = nullptr;          // split-FP16 Hadamard: largest |spectrum value| (float bits), else null
 };
 
 // fft_up_equiv: one (source slot, child) per work item; src_node[slot] gives
@@ -100,12 +101,17 @@ struct HadBlocks {
   const short* loc = nullptr;
 };
 // m2l_opsx: operators of the level in the expanded MFMA form of launch_m2l_expand
-void launch_hadamard(const M2LDev& m, const void* m2l_opsx, const HadBlocks& hb, hipStream_t s);
+// hscale != null (mixed, with m.amax set): split-FP16 operators of
+// launch_m2l_expand with their per-frequency scales, spectra scaled by *m.amax
+void launch_hadamard(const M2LDev& m, const void* m2l_opsx, const float* hscale, const HadBlocks& hb,
+                     hipStream_t s);
 // Expand nlev operator levels from the reference layout [L][26][nfreq][64]
 // complex into the per-lane MFMA A-operand form [L][nfreq][26][64 lanes][4]
 // (float for the mixed path, else double), frequencies in the order of the
 // transforms that m selects.
-void launch_m2l_expand(const M2LDev& m, int nlev, const void* M, void* X, hipStream_t s);
+// With hscale != null (mixed only) the split-FP16 form of k_hadamard_h
+// instead (same size), and hscale[L][k] receives the operator scales.
+void launch_m2l_expand(const M2LDev& m, int nlev, const void* M, void* X, float* hscale, hipStream_t s);
 // ifft_dn_check: one (target slot, child) per work item, adds
 // scale[slot] * result to the child's dn vector.
 void launch_ifft_dn(const M2LDev& m, const DevTree& t, const int* trg_node, const double* scale, int nslots,

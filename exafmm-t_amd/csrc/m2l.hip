@@ -28,6 +28,7 @@ namespace xfmm {
 
 template <typename R>
 struct C2 { R x, y; };
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 
 template <typename R>
 __device__ __forceinline__ C2<R> cmul(C2<R> a, C2<R> b) {
@@ -59,6 +60,15 @@ __device__ __forceinline__ void xcd_slot_child(int w, int nslots, int& slot, int
   child = rem / 8;
   slot = grp * 8 + rem % 8;
   (void)nslots;
+}
+
+// Largest |value| a thread stored, reduced over the wave and folded into the
+// evaluation-wide maximum *amax (float bits; order-preserving for values
+// >= 0).  Called by every lane of the wave.
+__device__ __forceinline__ void wave_amax(float v, unsigned* amax) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(amax, __float_as_uint(v));
 }
 
 template <typename R, bool CPLX>
@@ -108,12 +118,15 @@ __global__ __launch_bounds__(256) void k_fft_up(M2LDev m, DevTree t, const int* 
   __syncthreads();
   // z pass straight to global: f = kx + nh*(ky + n1*kz)
   C2<R>* out = reinterpret_cast<C2<R>*>(m.fft_in) + size_t(slot) * m.nfreq * NCHILD;
+  float mx = 0.f;
   for (int f = threadIdx.x; f < m.nfreq; f += blockDim.x) {
     const int kx = f % nh, ky = (f / nh) % n1, kz = f / (nh * n1);
     C2<R> acc{0, 0};
     for (int z = 0; z < p; z++) cfma(acc, B[(z * n1 + ky) * nh + kx], w[(kz * z) % n1]);
     out[size_t(f) * NCHILD + c] = acc;
+    mx = fmaxf(mx, fmaxf(fabsf(float(acc.x)), fabsf(float(acc.y))));
   }
+  if (m.amax) wave_amax(mx, m.amax);
 }
 
 template <typename R, bool CPLX>
@@ -313,11 +326,69 @@ __global__ void k_m2l_expand(int nlev, int nfreq, int n1, bool slab, const R* __
   }
 }
 
-void launch_m2l_expand(const M2LDev& m, int nlev, const void* M, void* X, hipStream_t s) {
+// Split-FP16 operator form for the mixed Hadamard (k_hadamard_h).  Per
+// operator level L and frequency k the 26 relative positions are taken in
+// pairs (2 pm, 2 pm + 1); for pair pm and lane (g, j) of a wave the 8 A
+// values of a 16x16x32 MFMA are, for c' = 0..3 and source child
+// s = 4 (g & 1) + c' of relative position r = 2 pm + (g >> 1), t = j & 7:
+//   j <  8: ( Re M[s][t], -Im M[s][t] )      (Re of target child t)
+//   j >= 8: ( Im M[s][t],  Re M[s][t] )      (Im of target child t)
+// scaled by 2^e(L,k) so that the largest |entry| of (L, k) lies in
+// [2^14, 2^15), and split a = hi + lo, hi = fp16(a), lo = fp16(a - hi).
+// Stored as X[((L nfreq + k) 13 + pm) 2 + term][lane][8] (16 bytes per lane);
+// hscale[L][k] = 2^-e(L,k).
+__global__ __launch_bounds__(256) void k_m2l_expand_h(int nfreq, int n1, bool slab, const float* __restrict__ M,
+                                                      h8* __restrict__ X, float* __restrict__ hscale) {
+  __shared__ float red[256];
+  const int lk = blockIdx.x, k = lk % nfreq, L = lk / nfreq, tid = threadIdx.x;
+  int f = k;
+  if (slab) {
+    const int kx = k / (n1 * n1), rem = k % (n1 * n1);
+    f = kx + (n1 / 2 + 1) * rem;
+  }
+  auto src = [&](int r) { return M + ((size_t(L) * NM2L + r) * nfreq + f) * 128; };
+  float mx = 0.f;
+  for (int e = tid; e < NM2L * 128; e += 256) mx = fmaxf(mx, fabsf(src(e >> 7)[e & 127]));
+  red[tid] = mx;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) red[tid] = fmaxf(red[tid], red[tid + w]);
+    __syncthreads();
+  }
+  int ex = 0;
+  frexpf(red[0], &ex);
+  if (!(red[0] <= 3.0e38f)) ex = 15;
+  const float sc = ldexpf(1.f, 15 - ex);
+  if (tid == 0) hscale[lk] = ldexpf(1.f, ex - 15);
+  h8* o = X + size_t(lk) * (NM2L / 2) * 2 * 64;
+  for (int e = tid; e < (NM2L / 2) * 64; e += 256) {
+    const int pm = e >> 6, lane = e & 63, g = lane >> 4, j = lane & 15, t = j & 7;
+    const float* mr = src(2 * pm + (g >> 1));
+    h8 hi, lo;
+#pragma unroll
+    for (int cp = 0; cp < 4; cp++) {
+      const int sidx = 4 * (g & 1) + cp;
+      const float re = mr[2 * (sidx * 8 + t)], im = mr[2 * (sidx * 8 + t) + 1];
+      const float a0 = sc * (j < 8 ? re : im), a1 = sc * (j < 8 ? -im : re);
+      const _Float16 h0 = _Float16(a0), h1 = _Float16(a1);
+      hi[2 * cp] = h0;
+      hi[2 * cp + 1] = h1;
+      lo[2 * cp] = _Float16(a0 - float(h0));
+      lo[2 * cp + 1] = _Float16(a1 - float(h1));
+    }
+    o[(pm * 2) * 64 + lane] = hi;
+    o[(pm * 2 + 1) * 64 + lane] = lo;
+  }
+}
+
+void launch_m2l_expand(const M2LDev& m, int nlev, const void* M, void* X, float* hscale, hipStream_t s) {
   const size_t n = size_t(nlev) * m.nfreq * NM2L * 64;
   const unsigned blocks = unsigned(std::min<size_t>((n + 255) / 256, 65536));
   const bool slab = fft_node_ok(m);
-  if (m.mixed)
+  if (m.mixed && hscale)
+    hipLaunchKernelGGL(k_m2l_expand_h, dim3(unsigned(nlev * m.nfreq)), dim3(256), 0, s, m.nfreq, m.n1, slab,
+                       static_cast<const float*>(M), static_cast<h8*>(X), hscale);
+  else if (m.mixed)
     hipLaunchKernelGGL(k_m2l_expand<float>, dim3(blocks), dim3(256), 0, s, nlev, m.nfreq, m.n1, slab,
                        static_cast<const float*>(M), static_cast<float*>(X));
   else
@@ -448,6 +519,125 @@ __global__ __launch_bounds__(256) void k_hadamard(M2LDev m, const R* __restrict_
   }
 }
 
+// Mixed-precision Hadamard on the FP16 matrix cores.  Every FP32 operand a is
+// carried as two FP16 numbers, a = hi + lo (22 significant bits), and a
+// product as hi_a hi_b + hi_a lo_b + lo_a hi_b (three v_mfma_f32_16x16x32_f16
+// with FP32 accumulation; the dropped lo_a lo_b is 2^-22 relative).  FP16
+// range is kept by power-of-two scales: the operators per (level,
+// frequency) at expansion time, the spectra by one scale per evaluation
+// from the largest |spectrum value| that the forward transforms recorded in
+// *m.amax; the result is multiplied back by both before it is stored.
+// One 16x16x32 MFMA covers two relative positions: lane groups g = 0, 1
+// take source children 0..3 / 4..7 of position 2 pm, groups 2, 3 those of
+// 2 pm + 1 (A layout of k_m2l_expand_h); the B operand of a lane is the
+// matching 4 children (re, im) of its target's source, staged in LDS as
+// [term][h][union slot] 16-byte vectors.  Blocks, union staging, grid order
+// and result layout are those of k_hadamard.
+__global__ __launch_bounds__(256) void k_hadamard_h(M2LDev m, const h8* __restrict__ X,
+                                                    const float* __restrict__ hscale, HadBlocks hb, int nchunk) {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  constexpr int US = HAD_UCAP + 1;
+  constexpr int NP = NM2L / 2;
+  __shared__ h8 Us[4 * US];        // [term][h][union slot]; slot HAD_UCAP stays zero
+  __shared__ h8 Ms[NP * 2 * 64];   // [pair][term][lane]
+  const int w = blockIdx.x;
+  const int x = w & 7, i = w >> 3;
+  const int chunk = (i / hb.nblk) * 8 + x;
+  const int b = i % hb.nblk;
+  if (chunk >= nchunk) return;
+  const int kbeg = chunk * HAD_KCH, kend = min(m.nfreq, kbeg + HAD_KCH);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int j = lane & 15, g = lane >> 4;
+  const int nu = hb.nunion[b];
+  if (tid < 4) Us[tid * US + HAD_UCAP] = h8{0, 0, 0, 0, 0, 0, 0, 0};
+  int uoff[NP];
+#pragma unroll
+  for (int pm = 0; pm < NP; pm++) {
+    const int u = hb.loc[(size_t(b) * NM2L + 2 * pm + (g >> 1)) * 64 + wave * 16 + j];
+    uoff[pm] = (g & 1) * US + (u >= 0 ? u : HAD_UCAP);
+  }
+  // spectrum scale: largest |value| of this evaluation's spectra -> [2^14, 2^15)
+  const float amax = __uint_as_float(*m.amax);
+  int ex = 0;
+  frexpf(amax, &ex);
+  if (!(amax <= 3.0e38f)) ex = 15;
+  const float sb = ldexpf(1.f, 15 - ex), isb = ldexpf(1.f, ex - 15);
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  const f4* fin = reinterpret_cast<const f4*>(m.fft_in);  // [slot][k][2 halves of 4 children]
+  C2<float>* fout = reinterpret_cast<C2<float>*>(m.fft_out);
+  // staging: element e = tid + 256 q is union slot e / 2, children 4 (e % 2) .. + 3
+  constexpr int NU_Q = HAD_UCAP * 2 / 256;
+  const int hh = tid & 1;
+  int usrc[NU_Q];
+#pragma unroll
+  for (int q = 0; q < NU_Q; q++) {
+    const int e = tid + 256 * q;
+    usrc[q] = (e < nu * 2) ? hb.uni[size_t(b) * HAD_UCAP + (e >> 1)] : -1;
+  }
+  constexpr int NM_Q = (NP * 2 * 64 + 255) / 256;
+  f4 ra[NU_Q], rb[NU_Q];
+  h8 rm[NM_Q];
+  auto fetch = [&](int k) {
+#pragma unroll
+    for (int q = 0; q < NU_Q; q++)
+      if (usrc[q] >= 0) {
+        const f4* f = fin + ((size_t(usrc[q]) * m.nfreq + k) * 2 + hh) * 2;
+        ra[q] = f[0];
+        rb[q] = f[1];
+      }
+    const h8* xk = X + size_t(k) * NP * 2 * 64;
+#pragma unroll
+    for (int q = 0; q < NM_Q; q++)
+      if (tid + 256 * q < NP * 2 * 64) rm[q] = xk[tid + 256 * q];
+  };
+  const int trg = hb.trg[size_t(b) * 64 + wave * 16 + j];
+  fetch(kbeg);
+  for (int k = kbeg; k < kend; k++) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NU_Q; q++)
+      if (usrc[q] >= 0) {
+        h8 hi, lo;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          const float v = sb * (e < 4 ? ra[q][e] : rb[q][e - 4]);
+          const _Float16 h = _Float16(v);
+          hi[e] = h;
+          lo[e] = _Float16(v - float(h));
+        }
+        const int slot = (tid + 256 * q) >> 1;
+        Us[hh * US + slot] = hi;
+        Us[2 * US + hh * US + slot] = lo;
+      }
+#pragma unroll
+    for (int q = 0; q < NM_Q; q++)
+      if (tid + 256 * q < NP * 2 * 64) Ms[tid + 256 * q] = rm[q];
+    __syncthreads();
+    if (k + 1 < kend) fetch(k + 1);
+    v4 ahh = {0, 0, 0, 0}, ahl = {0, 0, 0, 0}, alh = {0, 0, 0, 0};
+#pragma unroll
+    for (int pm = 0; pm < NP; pm++) {
+      const h8 ah = Ms[(2 * pm) * 64 + lane], al = Ms[(2 * pm + 1) * 64 + lane];
+      const h8 bh = Us[uoff[pm]], bl = Us[2 * US + uoff[pm]];
+      ahh = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, ahh, 0, 0, 0);
+      ahl = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, ahl, 0, 0, 0);
+      alh = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, alh, 0, 0, 0);
+    }
+    const float sc = hscale[k] * isb;
+    const v4 acc = (ahh + (ahl + alh)) * sc;
+    // lane (g, j) register q holds row 4 g + q of target j: rows 0..7 Re of
+    // children 0..7, rows 8..15 Im
+    if (trg >= 0) {
+      float* o = reinterpret_cast<float*>(fout + (size_t(trg) * m.nfreq + k) * NCHILD);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int rw = 4 * g + q;
+        o[2 * (rw & 7) + (rw >> 3)] = acc[q];
+      }
+    }
+  }
+}
+
 // ------------------------------------------------ per-node MFMA transforms
 //
 // One workgroup per M2L node (its 8 children), in kx slabs.  Each 1-D DFT
@@ -530,6 +720,7 @@ __global__ __launch_bounds__(256) void k_fft_up_mfma(M2LDev m, DevTree t, const 
     }
   __syncthreads();
   C2<RS>* out = reinterpret_cast<C2<RS>*>(m.fft_in) + size_t(slot) * m.nfreq * NCHILD;
+  float mx = 0.f;
   for (int kx = 0; kx < nh; kx++) {
     // x: Ay[(y, re/im)][(z, c)] = sum_x gx[c][z][y][x] exp(-2 pi i kx x / n1)
     C2<R> tw[P];
@@ -588,13 +779,18 @@ __global__ __launch_bounds__(256) void k_fft_up_mfma(M2LDev m, DevTree t, const 
 #pragma unroll
         for (int h = 0; h < 2; h++) {
           const int kz = mt * 8 + RowMap<R>::fd(g, 2 * h);
-          if (kz < n1) out[size_t(kx * n1 * n1 + ky + n1 * kz) * NCHILD + c] = C2<RS>{RS(acc[2 * h]), RS(acc[2 * h + 1])};
+          if (kz < n1) {
+            const C2<RS> v{RS(acc[2 * h]), RS(acc[2 * h + 1])};
+            out[size_t(kx * n1 * n1 + ky + n1 * kz) * NCHILD + c] = v;
+            mx = fmaxf(mx, fmaxf(fabsf(float(v.x)), fabsf(float(v.y))));
+          }
         }
       }
     }
     // the next x pass writes Ay only: its last readers (y pass) are behind
     // the barrier above
   }
+  if (m.amax) wave_amax(mx, m.amax);
 }
 
 // ifft_dn_check for one node: dn[c][i] += scale * sum over (kx, ky, kz) of
@@ -849,12 +1045,15 @@ void launch_ifft_dn(const M2LDev& m, const DevTree& t, const int* trg_node, cons
   }
 }
 
-void launch_hadamard(const M2LDev& m, const void* m2l_opsx, const HadBlocks& hb, hipStream_t s) {
+void launch_hadamard(const M2LDev& m, const void* m2l_opsx, const float* hscale, const HadBlocks& hb,
+                     hipStream_t s) {
   if (hb.nblk <= 0) return;
   const int nchunk = (m.nfreq + HAD_KCH - 1) / HAD_KCH;
   const int nchunk8 = (nchunk + 7) / 8 * 8;
   dim3 grid(unsigned(nchunk8) * unsigned(hb.nblk));
-  if (m.mixed)
+  if (m.mixed && hscale && m.amax)
+    hipLaunchKernelGGL(k_hadamard_h, grid, dim3(256), 0, s, m, static_cast<const h8*>(m2l_opsx), hscale, hb, nchunk);
+  else if (m.mixed)
     hipLaunchKernelGGL((k_hadamard<float>), grid, dim3(256), 0, s, m, static_cast<const float*>(m2l_opsx), hb, nchunk);
   else
     hipLaunchKernelGGL((k_hadamard<double>), grid, dim3(256), 0, s, m, static_cast<const double*>(m2l_opsx), hb, nchunk);

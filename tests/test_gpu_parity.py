@@ -137,6 +137,32 @@ def test_mixed_precision(require_gpu, kernel, p, dist, n, ncrit):
     assert emx < max(2 * e64, 1e-5)
 
 
+@pytest.mark.parametrize("kernel,p,dist", [("laplace", 10, "clusters"), ("helmholtz", 5, "cube")])
+def test_mixed_hadamard_fp16_vs_fp32(require_gpu, monkeypatch, kernel, p, dist):
+    """The default mixed Hadamard (split FP16 on the f16 matrix cores) and the
+    plain FP32 one (selected with XFMM_MIXED_STAGES) both stay within the
+    mixed tolerance of the FP64 evaluation, and close to each other."""
+    pr = make_problem(kernel, 20000, p, ncrit=64, dist=dist, seed=11)
+    print("hadamard variants", kernel, p, dist, flush=True)
+    out = {}
+    for name, stages in (("fp64", None), ("had16", "p2p,m2l,had16,fft"), ("had32", "p2p,m2l,fft")):
+        if stages is None:
+            monkeypatch.delenv("XFMM_MIXED_STAGES", raising=False)
+        else:
+            monkeypatch.setenv("XFMM_MIXED_STAGES", stages)
+        eng = _engine(pr, abi.FP64 if stages is None else abi.MIXED)
+        eng.upward_pass()
+        eng.downward_pass()
+        out[name] = eng.values()
+    monkeypatch.delenv("XFMM_MIXED_STAGES", raising=False)
+    for name in ("had16", "had32"):
+        e0 = rel_err(out[name][:, 0], out["fp64"][:, 0])
+        e1 = rel_err(out[name][:, 1:], out["fp64"][:, 1:])
+        print(name, "phi %.2e grad %.2e" % (e0, e1))
+        assert e0 < 1e-5 and e1 < 1e-4, (name, e0, e1)
+    assert rel_err(out["had16"][:, 0], out["had32"][:, 0]) < 1e-5
+
+
 def test_python_api_evaluate_and_verify(require_gpu):
     import exafmm
     L = exafmm.laplace
