@@ -789,30 +789,67 @@ int xfmm_set_tree(xfmm_context* c, const xfmm_tree* t) {
         srcs.push_back(i);
       }
     // blocks: consecutive targets of one level (Hilbert order), at most 64
-    // targets and HAD_UCAP distinct sources each
+    // targets and HAD_UROWS union rows per colour each (slot and lane
+    // placement rules at HAD_UCAP)
+    auto lattice = [&](int node, int* ijk) {
+      const double h = 2.0 * t->r0 / double(1 << lev[node]);
+      for (int d = 0; d < 3; d++) {
+        const double v = std::floor((t->center[3 * size_t(node) + d] - (t->x0[d] - t->r0)) / h);
+        ijk[d] = int(std::max(0.0, v));
+      }
+    };
+    auto colour = [&](int node) {
+      int q[3];
+      lattice(node, q);
+      return (q[0] + 4 * q[1] + 8 * q[2]) & 15;
+    };
+    std::vector<int> src_colour(srcs.size());
+    for (size_t k = 0; k < srcs.size(); k++) src_colour[k] = colour(srcs[k]);
     std::vector<int> btrg, buni, bnu;
     std::vector<short> bloc;
     c->blk_level_off.assign(D + 2, 0);
     int nblk = 0;
-    std::vector<int> lidx(std::max<size_t>(1, srcs.size()), -1);  // src slot -> index in current union
+    std::vector<int> lidx(std::max<size_t>(1, srcs.size()), -1);  // src slot -> union slot in current block
     std::vector<int> cur_t, cur_u;
+    int rows[16] = {0};
     auto flush = [&]() {
       if (cur_t.empty()) return;
       const size_t bt = btrg.size(), bu = buni.size(), bl = bloc.size();
       btrg.resize(bt + 64, -1);
-      buni.resize(bu + HAD_UCAP, 0);
+      buni.resize(bu + HAD_UCAP, -1);
       bloc.resize(bl + size_t(NM2L) * 64, short(-1));
-      for (size_t u = 0; u < cur_u.size(); u++) buni[bu + u] = cur_u[u];
+      int nu = 0;
+      for (int u : cur_u) {
+        buni[bu + lidx[u]] = u;
+        nu = std::max(nu, lidx[u] + 1);
+      }
+      // lane placement: preferred position, else the first free one
+      std::vector<int> pos(cur_t.size(), -1);
+      bool used[64] = {false};
+      for (size_t jt = 0; jt < cur_t.size(); jt++) {
+        int q[3];
+        lattice(cur_t[jt], q);
+        const int want = 16 * (((q[1] >> 1) & 1) + 2 * ((q[2] >> 1) & 1)) + (q[0] & 3) + 4 * (q[1] & 1) + 8 * (q[2] & 1);
+        if (!used[want]) { used[want] = true; pos[jt] = want; }
+      }
+      int nextfree = 0;
+      for (size_t jt = 0; jt < cur_t.size(); jt++) {
+        if (pos[jt] >= 0) continue;
+        while (used[nextfree]) nextfree++;
+        used[nextfree] = true;
+        pos[jt] = nextfree;
+      }
       for (size_t jt = 0; jt < cur_t.size(); jt++) {
         const int node = cur_t[jt];
-        btrg[bt + jt] = trg_slot[node];
+        btrg[bt + pos[jt]] = trg_slot[node];
         for (int r = 0; r < NM2L; r++) {
           const int sn = t->m2l_source[size_t(node) * NM2L + r];
-          if (sn >= 0) bloc[bl + size_t(r) * 64 + jt] = short(lidx[src_slot[sn]]);
+          if (sn >= 0) bloc[bl + size_t(r) * 64 + pos[jt]] = short(lidx[src_slot[sn]]);
         }
       }
       for (int u : cur_u) lidx[u] = -1;
-      bnu.push_back(int(cur_u.size()));
+      for (int& r : rows) r = 0;
+      bnu.push_back(nu);
       nblk++;
       cur_t.clear();
       cur_u.clear();
@@ -821,19 +858,24 @@ int xfmm_set_tree(xfmm_context* c, const xfmm_tree* t) {
       c->blk_level_off[l] = nblk;
       for (int i = lo[l]; i < lo[l + 1]; i++) {
         if (trg_slot[i] < 0) continue;
-        int extra = 0;
+        int extra[16] = {0};
+        bool over = false;
         for (int r = 0; r < NM2L; r++) {
           const int sn = t->m2l_source[size_t(i) * NM2L + r];
-          if (sn >= 0 && lidx[src_slot[sn]] < 0) extra++;  // (a source repeated over r is counted twice: conservative)
+          if (sn < 0 || lidx[src_slot[sn]] >= 0) continue;
+          // (a source repeated over r is counted twice: conservative)
+          const int col = src_colour[src_slot[sn]];
+          if (rows[col] + ++extra[col] > HAD_UROWS) over = true;
         }
-        if (cur_t.size() == 64 || int(cur_u.size()) + extra > HAD_UCAP) flush();
+        if (cur_t.size() == 64 || over) flush();
         cur_t.push_back(i);
         for (int r = 0; r < NM2L; r++) {
           const int sn = t->m2l_source[size_t(i) * NM2L + r];
           if (sn < 0) continue;
           const int ss = src_slot[sn];
           if (lidx[ss] < 0) {
-            lidx[ss] = int(cur_u.size());
+            const int col = src_colour[ss];
+            lidx[ss] = 16 * rows[col]++ + col;
             cur_u.push_back(ss);
           }
         }

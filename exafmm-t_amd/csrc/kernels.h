@@ -91,7 +91,14 @@ void launch_fft_up(const M2LDev& m, const DevTree& t, const int* src_node, int n
 // slots of the union of the block's sources (nunion[b] used), loc[b][26][64]
 // local union index of the source of target t for relative position r (-1 =
 // none).
-constexpr int HAD_UCAP = 256;  // max sources per block (LDS staging)
+// Union slots of a block: slot = 16 row + colour, colour = (x + 4 y + 8 z) mod 16
+// of the source's lattice position at its level, at most HAD_UROWS rows.
+// Targets sit at lane (x mod 4) + 4 (y mod 2) + 8 (z mod 2) of tile
+// (y / 2 mod 2) + 2 (z / 2 mod 2) where that is free, so that the 16 sources a
+// wave gathers for one relative position fall on 16 different colours (LDS
+// banks).
+constexpr int HAD_UROWS = 18;
+constexpr int HAD_UCAP = 16 * HAD_UROWS;  // union slots per block (LDS staging)
 constexpr int HAD_KCH = 16;    // frequencies per workgroup
 struct HadBlocks {
   int nblk = 0;

@@ -293,21 +293,38 @@ struct Quad<double> {
 
 // Operators in MFMA A-operand form: for operator level L, frequency k,
 // relative position r and lane (g, j) of a wave, the four A values of the
-// lane's four MFMAs (rows j < 8: Re of target child j, j >= 8: Im of child
-// j - 8; inner index g resp. g + 4 = source child):
-//   j <  8: { Re M[g][t],  Re M[g+4][t], -Im M[g][t], -Im M[g+4][t] }
-//   j >= 8: { Im M[g][t],  Im M[g+4][t],  Re M[g][t],  Re M[g+4][t] },  t = j & 7,
+// lane's four MFMAs (row j = (Re or Im, target child t) by HadRow, inner index
+// g resp. g + 4 = source child):
+//   Re row: { Re M[g][t],  Re M[g+4][t], -Im M[g][t], -Im M[g+4][t] }
+//   Im row: { Im M[g][t],  Im M[g+4][t],  Re M[g][t],  Re M[g+4][t] },
 // stored as X[((L nfreq + k) 26 + r) 64 + 16 g + j], so a wave reads one
 // contiguous 1 KiB per (k, r).  Built once from the reference layout
 // M[L][r][f][8 s + t] (complex, f = kx + (p+1) (ky + 2p kz)).  With
 // slab = true, X (and the spectra of the per-node transforms) number the
 // frequencies kx-slab first, k = kx (2p)^2 + ky + 2p kz, so that one kx slab
 // of a node's spectrum is contiguous in memory.
+// Row j of the 16-row Hadamard product: target child and Re (0) / Im (1),
+// arranged so that the result registers of a lane hold whole (Re, Im) pairs:
+// FP32 (register q of group g = row 4 g + q): children 2g, 2g+1, one 16-byte
+// store; FP64 (row g + 4 q): children g and g + 4, two 16-byte stores.
+template <typename R>
+struct HadRow;
+template <>
+struct HadRow<float> {
+  static __device__ __forceinline__ int child(int j) { return j >> 1; }
+  static __device__ __forceinline__ int im(int j) { return j & 1; }
+};
+template <>
+struct HadRow<double> {
+  static __device__ __forceinline__ int child(int j) { return (j & 3) + 4 * (j >> 3); }
+  static __device__ __forceinline__ int im(int j) { return (j >> 2) & 1; }
+};
+
 template <typename R>
 __global__ void k_m2l_expand(int nlev, int nfreq, int n1, bool slab, const R* __restrict__ M, R* __restrict__ X) {
   const size_t n = size_t(nlev) * nfreq * NM2L * 64;
   for (size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += size_t(gridDim.x) * blockDim.x) {
-    const int lane = int(e % 64), g = lane >> 4, j = lane & 15, t = j & 7;
+    const int lane = int(e % 64), g = lane >> 4, j = lane & 15, t = HadRow<R>::child(j);
     const size_t q = e / 64;
     const int r = int(q % NM2L);
     const size_t lk = q / NM2L;  // L * nfreq + k
@@ -321,7 +338,7 @@ __global__ void k_m2l_expand(int nlev, int nfreq, int n1, bool slab, const R* __
     const R m0x = src[2 * (g * 8 + t)], m0y = src[2 * (g * 8 + t) + 1];
     const R m1x = src[2 * ((g + 4) * 8 + t)], m1y = src[2 * ((g + 4) * 8 + t) + 1];
     R* o = X + e * 4;
-    if (j < 8) { o[0] = m0x; o[1] = m1x; o[2] = -m0y; o[3] = -m1y; }
+    if (!HadRow<R>::im(j)) { o[0] = m0x; o[1] = m1x; o[2] = -m0y; o[3] = -m1y; }
     else { o[0] = m0y; o[1] = m1y; o[2] = m0x; o[3] = m1x; }
   }
 }
@@ -330,9 +347,9 @@ __global__ void k_m2l_expand(int nlev, int nfreq, int n1, bool slab, const R* __
 // operator level L and frequency k the 26 relative positions are taken in
 // pairs (2 pm, 2 pm + 1); for pair pm and lane (g, j) of a wave the 8 A
 // values of a 16x16x32 MFMA are, for c' = 0..3 and source child
-// s = 4 (g & 1) + c' of relative position r = 2 pm + (g >> 1), t = j & 7:
-//   j <  8: ( Re M[s][t], -Im M[s][t] )      (Re of target child t)
-//   j >= 8: ( Im M[s][t],  Re M[s][t] )      (Im of target child t)
+// s = 4 (g & 1) + c' of relative position r = 2 pm + (g >> 1), t = j >> 1:
+//   j even: ( Re M[s][t], -Im M[s][t] )      (Re of target child t)
+//   j odd:  ( Im M[s][t],  Re M[s][t] )      (Im of target child t)
 // scaled by 2^e(L,k) so that the largest |entry| of (L, k) lies in
 // [2^14, 2^15), and split a = hi + lo, hi = fp16(a), lo = fp16(a - hi).
 // Stored as X[((L nfreq + k) 13 + pm) 2 + term][lane][8] (16 bytes per lane);
@@ -362,14 +379,15 @@ __global__ __launch_bounds__(256) void k_m2l_expand_h(int nfreq, int n1, bool sl
   if (tid == 0) hscale[lk] = ldexpf(1.f, ex - 15);
   h8* o = X + size_t(lk) * (NM2L / 2) * 2 * 64;
   for (int e = tid; e < (NM2L / 2) * 64; e += 256) {
-    const int pm = e >> 6, lane = e & 63, g = lane >> 4, j = lane & 15, t = j & 7;
+    const int pm = e >> 6, lane = e & 63, g = lane >> 4, j = lane & 15, t = j >> 1;
     const float* mr = src(2 * pm + (g >> 1));
     h8 hi, lo;
 #pragma unroll
     for (int cp = 0; cp < 4; cp++) {
       const int sidx = 4 * (g & 1) + cp;
       const float re = mr[2 * (sidx * 8 + t)], im = mr[2 * (sidx * 8 + t) + 1];
-      const float a0 = sc * (j < 8 ? re : im), a1 = sc * (j < 8 ? -im : re);
+      const bool re_row = (j & 1) == 0;
+      const float a0 = sc * (re_row ? re : im), a1 = sc * (re_row ? -im : re);
       const _Float16 h0 = _Float16(a0), h1 = _Float16(a1);
       hi[2 * cp] = h0;
       hi[2 * cp + 1] = h1;
@@ -406,8 +424,8 @@ void launch_m2l_expand(const M2LDev& m, int nlev, const void* M, void* X, float*
 // frequency's data is loaded into registers while the current one computes.
 // Each wave owns 16 targets; per
 // relative position the complex 8x8 product is one real 16x16x16 product
-// (four 16x16x4 MFMAs): rows 0..7 of the result = Re of target children,
-// rows 8..15 = Im.  A lane's union index for each relative position is fixed
+// (four 16x16x4 MFMAs) whose rows are (Re or Im, target child) in HadRow
+// order.  A lane's union index for each relative position is fixed
 // for the block and held in registers; an absent source points at a zero
 // slot, so the 26 relative positions run without branches.
 template <typename R>
@@ -443,7 +461,7 @@ __global__ __launch_bounds__(256) void k_hadamard(M2LDev m, const R* __restrict_
   const q4* X = reinterpret_cast<const q4*>(opsx);
   // staging: element e = tid + 256 q (q < 4) is slot e / 4, pair g' = e % 4
   // (children g' and g' + 4)
-  constexpr int NU_Q = HAD_UCAP * 4 / 256;
+  constexpr int NU_Q = (HAD_UCAP * 4 + 255) / 256;
   const int gp = tid & 3;
   int usrc[NU_Q];
 #pragma unroll
@@ -506,15 +524,12 @@ __global__ __launch_bounds__(256) void k_hadamard(M2LDev m, const R* __restrict_
       vv = vn;
     }
     const v4 acc = acc0 + acc1;
-    // result: lane (g, j) register q holds row MfmaT::row(g, q) of target j;
-    // rows 0..7 are Re of children 0..7, rows 8..15 Im.
+    // result: lane (g, j) register q holds row MfmaT::row(g, q) of target j,
+    // registers (0, 1) and (2, 3) the (Re, Im) of one child each (HadRow)
     if (trg >= 0) {
-      R* o = reinterpret_cast<R*>(fout + (size_t(trg) * m.nfreq + k) * NCHILD);
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int rw = MfmaT<R>::row(g, q);
-        o[2 * (rw & 7) + (rw >> 3)] = acc[q];
-      }
+      C2<R>* o = fout + (size_t(trg) * m.nfreq + k) * NCHILD;
+      o[HadRow<R>::child(MfmaT<R>::row(g, 0))] = C2<R>{acc[0], acc[1]};
+      o[HadRow<R>::child(MfmaT<R>::row(g, 2))] = C2<R>{acc[2], acc[3]};
     }
   }
 }
@@ -566,7 +581,7 @@ __global__ __launch_bounds__(256) void k_hadamard_h(M2LDev m, const h8* __restri
   const f4* fin = reinterpret_cast<const f4*>(m.fft_in);  // [slot][k][2 halves of 4 children]
   C2<float>* fout = reinterpret_cast<C2<float>*>(m.fft_out);
   // staging: element e = tid + 256 q is union slot e / 2, children 4 (e % 2) .. + 3
-  constexpr int NU_Q = HAD_UCAP * 2 / 256;
+  constexpr int NU_Q = (HAD_UCAP * 2 + 255) / 256;
   const int hh = tid & 1;
   int usrc[NU_Q];
 #pragma unroll
@@ -625,16 +640,9 @@ __global__ __launch_bounds__(256) void k_hadamard_h(M2LDev m, const h8* __restri
     }
     const float sc = hscale[k] * isb;
     const v4 acc = (ahh + (ahl + alh)) * sc;
-    // lane (g, j) register q holds row 4 g + q of target j: rows 0..7 Re of
-    // children 0..7, rows 8..15 Im
-    if (trg >= 0) {
-      float* o = reinterpret_cast<float*>(fout + (size_t(trg) * m.nfreq + k) * NCHILD);
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int rw = 4 * g + q;
-        o[2 * (rw & 7) + (rw >> 3)] = acc[q];
-      }
-    }
+    // lane (g, j) register q holds row 4 g + q of target j: (Re, Im) of
+    // children 2g and 2g + 1, one 16-byte store
+    if (trg >= 0) *reinterpret_cast<v4*>(fout + (size_t(trg) * m.nfreq + k) * NCHILD + 2 * g) = acc;
   }
 }
 
