@@ -116,6 +116,8 @@ struct xfmm_context {
   // work lists
   DevBuf<int> w_p2p, w_m2p, w_p2l, w_l2p_eval, w_leaf;  // w_leaf: P2M/L2P conversion leaves, sorted by level
   int n_p2p = 0, n_m2p = 0, n_p2l = 0, n_l2p_eval = 0, n_leaf = 0;
+  DevBuf<int> t_p2p, t_l2p;  // (leaf, 64-target tile) pairs of w_p2p / w_l2p_eval
+  int nt_p2p = 0, nt_l2p = 0;
   std::vector<int> leaf_level_off;                        // [depth+2] ranges in w_leaf
   DevBuf<long> leaf_slot_rows, leaf_node_rows;            // i*nsurf, node*nsurf
   std::vector<int> nl_level_off;                          // [depth+2] ranges of the non-leaf list
@@ -129,7 +131,7 @@ struct xfmm_context {
   DevBuf<char> fft_in, fft_out;
   // buffers
   DevBuf<double> up, dn, trg_value, check, tmp, gemm_scratch, near;
-  DevBuf<double> src_rel, src_pack;  // P2P sources relative to their leaf centre; packed (x, y, z, q)
+  DevBuf<double> src_rel, src_pack, surf_tab;  // P2P sources relative to their leaf centre; packed (x, y, z, q)
 
   // ---- timing
   hipEvent_t ev[24] = {};
@@ -166,6 +168,7 @@ struct xfmm_context {
     t.unit_surf = unit_surf.ptr;
     t.nsrcs = nsrcs;
     t.src_rel = src_rel.ptr;
+    t.surf_tab = surf_tab.ptr;
     t.src_pack = src_pack.ptr;
     t.up = up.ptr;
     t.dn = dn.ptr;
@@ -346,8 +349,8 @@ void op_L2P(xfmm_context* c) {
                      c->stream);
   }
   DevTree t = c->dev_tree();
-  launch_eval_targets(c->kind, c->f32_far, MODE_L2P, t, c->wavek(), c->w_l2p_eval.ptr, c->n_l2p_eval,
-                      c->stream);
+  launch_eval_targets(c->kind, c->f32_far, MODE_L2P, t, c->wavek(), c->w_l2p_eval.ptr, c->n_l2p_eval, c->t_l2p.ptr,
+                      c->nt_l2p, c->stream);
 }
 
 void op_P2L(xfmm_context* c) {
@@ -358,12 +361,13 @@ void op_P2L(xfmm_context* c) {
 
 void op_M2P(xfmm_context* c) {
   DevTree t = c->dev_tree();
-  launch_eval_targets(c->kind, c->f32_far, MODE_M2P, t, c->wavek(), c->w_m2p.ptr, c->n_m2p, c->stream);
+  launch_eval_targets(c->kind, c->f32_far, MODE_M2P, t, c->wavek(), c->w_m2p.ptr, c->n_m2p, nullptr, 0, c->stream);
 }
 
 void op_P2P(xfmm_context* c) {
   DevTree t = c->dev_tree();
-  launch_eval_targets(c->kind, c->f32_p2p, MODE_P2P, t, c->wavek(), c->w_p2p.ptr, c->n_p2p, c->stream);
+  launch_eval_targets(c->kind, c->f32_p2p, MODE_P2P, t, c->wavek(), c->w_p2p.ptr, c->n_p2p, c->t_p2p.ptr,
+                      c->nt_p2p, c->stream);
 }
 
 void record(xfmm_context* c, int i) {
@@ -380,7 +384,8 @@ void launch_near(xfmm_context* c) {
   if (!c->capturing) HIPCHK(hipEventRecord(c->ev[21], c->stream2));
   DevTree t = c->dev_tree();
   t.trg_value = c->near.ptr;
-  launch_eval_targets(c->kind, c->f32_p2p, MODE_P2P, t, c->wavek(), c->w_p2p.ptr, c->n_p2p, c->stream2);
+  launch_eval_targets(c->kind, c->f32_p2p, MODE_P2P, t, c->wavek(), c->w_p2p.ptr, c->n_p2p, c->t_p2p.ptr,
+                      c->nt_p2p, c->stream2);
   if (!c->capturing) HIPCHK(hipEventRecord(c->ev[22], c->stream2));
   HIPCHK(hipEventRecord(c->ev[20], c->stream2));
   c->near_pending = true;
@@ -703,6 +708,15 @@ int xfmm_set_tree(xfmm_context* c, const xfmm_tree* t) {
       }
       c->src_rel.upload(rel, st);
       c->src_pack.alloc(std::max<size_t>(1, size_t(t->nsrcs) * 4));
+      // L2P equivalent surface per level, relative to the box centre
+      const std::vector<double> us = unit_surface(c->p);
+      std::vector<double> tab(size_t(D + 1) * c->nsurf * 4, 0.0);
+      for (int l = 0; l <= D; l++) {
+        const double b = 2.95 * t->r0 * std::ldexp(1.0, -l);
+        for (int i = 0; i < c->nsurf; i++)
+          for (int d = 0; d < 3; d++) tab[(size_t(l) * c->nsurf + i) * 4 + d] = us[3 * size_t(i) + d] * b;
+      }
+      c->surf_tab.upload(tab, st);
     }
     c->trg.upload(t->trg_coord, size_t(t->ntrgs) * 3, st);
     c->p2p_off.upload(t->p2p_offset, nn + 1, st);
@@ -736,6 +750,20 @@ int xfmm_set_tree(xfmm_context* c, const xfmm_tree* t) {
       lsr[i] = long(i) * c->nsurf;
       lnr[i] = long(leaf[i]) * c->nsurf;
     }
+    auto tiles = [&](const std::vector<int>& w) {
+      std::vector<int> v;
+      for (int leaf : w)
+        for (int k = 0; 64 * k < t->trg_count[leaf]; k++) {
+          v.push_back(leaf);
+          v.push_back(k);
+        }
+      return v;
+    };
+    const std::vector<int> tp2p = tiles(wp2p), tl2p = tiles(wl2p);
+    c->t_p2p.upload(tp2p, st);
+    c->t_l2p.upload(tl2p, st);
+    c->nt_p2p = int(tp2p.size() / 2);
+    c->nt_l2p = int(tl2p.size() / 2);
     c->w_p2p.upload(wp2p, st);
     c->w_m2p.upload(wm2p, st);
     c->w_p2l.upload(wp2l, st);
@@ -1005,7 +1033,8 @@ int xfmm_downward_pass(xfmm_context* c) {
       DevTree t = c->dev_tree();
       t.trg_value = c->near.ptr;
       launch_zero(c->near.ptr, c->near.n, c->stream);
-      launch_eval_targets(c->kind, c->f32_p2p, MODE_P2P, t, c->wavek(), c->w_p2p.ptr, c->n_p2p, c->stream);
+      launch_eval_targets(c->kind, c->f32_p2p, MODE_P2P, t, c->wavek(), c->w_p2p.ptr, c->n_p2p, c->t_p2p.ptr,
+                      c->nt_p2p, c->stream);
     }
     record(c, 6);
     op_M2L(c, true);

@@ -30,6 +30,7 @@ struct DevTree {
   const int* p2l_src = nullptr;
   const double* unit_surf = nullptr;  // [nsurf][3] unit-cube surface (reference point order)
   const double* src_rel = nullptr;    // [nsrcs][3] source positions relative to their leaf centre
+  const double* surf_tab = nullptr;   // [depth + 1][nsurf][4] L2P equivalent surface (radius 2.95 r_l), (x, y, z, 0) relative to the centre
   void* src_pack = nullptr;           // [nsrcs] (x, y, z, q) relative to the leaf centre, in the P2P type
   double* up = nullptr;               // T[nnodes][nsurf]
   double* dn = nullptr;               // T[nnodes][nsurf]
@@ -42,9 +43,11 @@ enum SurfaceMode { MODE_P2M = 0, MODE_P2L = 1 };
 // Particle targets of the leaves in work[0..nwork): P2P (sources: leaf
 // particles of the P2P list), M2P (sources: upward equivalent surfaces of the
 // M2P list, alpha 1.05), L2P (source: the leaf's own downward equivalent
-// surface, alpha 2.95).  Adds grad/potential into trg_value.
+// surface, alpha 2.95).  Adds grad/potential into trg_value.  tiles[2 i],
+// tiles[2 i + 1] (i < ntiles) = (leaf, tile of 64 targets) of the same
+// leaves: the P2P and L2P kernels of the real kernels take one tile per wave.
 void launch_eval_targets(int kind, bool f32, int mode, const DevTree& t, Wavek k, const int* work, int nwork,
-                         hipStream_t s);
+                         const int* tiles, int ntiles, hipStream_t s);
 
 // Surface targets: P2M writes the upward check potential of each work leaf to
 // check[slot] (slot = position in work); P2L adds the potential of the P2L list

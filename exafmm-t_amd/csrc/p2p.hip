@@ -172,78 +172,198 @@ __global__ void k_pack_sources(int n, const double* __restrict__ rel, const doub
   out[i] = v;
 }
 
+// Tail of a leaf with nt <= TILE / 2 targets: k = TILE / nt lanes per target,
+// lane (target lane % nt, phase lane / nt) takes every k-th source from its
+// phase on (vector loads); the k partial sums of a target are added in FP64
+// through LDS by split_reduce.
+template <typename R, int NA>
+__device__ __forceinline__ void split_reduce(const R* acc, const R* comp, int nt, int k, double* out) {
+  __shared__ double red[TILE][NA];
+  const int lane = threadIdx.x;
+#pragma unroll
+  for (int a = 0; a < NA; a++) red[lane][a] = double(acc[a]) - double(comp[a]);
+  __syncthreads();
+  if (lane < nt) {
+#pragma unroll
+    for (int a = 0; a < NA; a++) {
+      double v = 0;
+      for (int ph = 0; ph < k; ph++) v += red[lane + ph * nt][a];
+      out[a] += v * FOUR_PI_INV;
+    }
+  }
+}
+
+// Near field of the real kernels (fmm_base.h:82-94): one wave per (leaf,
+// tile of TILE targets), work[2 b] = leaf, work[2 b + 1] = tile.  A full tile
+// has a target per lane and reads the packed sources (relative to their leaf
+// centre) with scalar loads (constant address space); a tail tile of at most
+// TILE / 2 targets splits the sources of each target over several lanes.
 template <int KIND, typename R>
 __global__ __launch_bounds__(64) void k_p2p_smem(DevTree t, Wavek wk, const int* __restrict__ work) {
   constexpr int NA = 4;
   // constant address space: the packed sources are read-only for the whole
   // launch, and wave-uniform loads from it become scalar loads
   typedef const R __attribute__((address_space(4))) * CR;
+  typedef typename P4<R>::t V;
   const CR src = (CR)(t.src_pack);
-  const int leaf = work[blockIdx.x];
+  const V* srcv = static_cast<const V*>(t.src_pack);
+  const int leaf = work[2 * blockIdx.x], t0 = TILE * work[2 * blockIdx.x + 1];
   const int lane = threadIdx.x;
   const double cx = t.center[3 * leaf], cy = t.center[3 * leaf + 1], cz = t.center[3 * leaf + 2];
-  const int tb = t.trg_begin[leaf], nt = t.trg_count[leaf];
+  const int tb = t.trg_begin[leaf], nt = min(TILE, t.trg_count[leaf] - t0);
   const R kr = R(wk.re), ki = R(wk.im);
   const int g0 = t.p2p_off[leaf], g1 = t.p2p_off[leaf + 1];
-  for (int t0 = 0; t0 < nt; t0 += TILE) {
-    const int ti = t0 + lane;
-    const bool act = ti < nt;
-    R tx = 0, ty = 0, tz = 0;
-    if (act) {
-      const double* x = t.trg + 3 * size_t(tb + ti);
-      tx = R(x[0] - cx);
-      ty = R(x[1] - cy);
-      tz = R(x[2] - cz);
-    }
-    R acc[NA], comp[NA];
+  const bool split = 2 * nt <= TILE;
+  const int k = split ? TILE / nt : 1;
+  const int ti = split ? lane % nt : lane, ph = split ? lane / nt : 0;
+  const bool act = split ? ph < k : ti < nt;
+  R tx = 0, ty = 0, tz = 0;
+  if (ti < nt) {
+    const double* x = t.trg + 3 * size_t(tb + t0 + ti);
+    tx = R(x[0] - cx);
+    ty = R(x[1] - cy);
+    tz = R(x[2] - cz);
+  }
+  R acc[NA], comp[NA];
 #pragma unroll
-    for (int a = 0; a < NA; a++) { acc[a] = 0; comp[a] = 0; }
-    for (int g = g0; g < g1; g++) {
-      const int snode = t.p2p_src[g];
-      const int sb = t.src_begin[snode], ns = t.src_count[snode];
-      // target in the frame of the source leaf
-      const R ux = tx - R(t.center[3 * snode] - cx);
-      const R uy = ty - R(t.center[3 * snode + 1] - cy);
-      const R uz = tz - R(t.center[3 * snode + 2] - cz);
-      const bool self = snode == leaf;
-      for (int s0 = 0; s0 < ns; s0 += TILE) {
-        const int m = min(TILE, ns - s0);
-        const CR S = src + 4 * size_t(sb + s0);
-        R part[NA] = {0, 0, 0, 0};
-        int j = 0;
-        if (self) {
-          for (; j + 4 <= m; j += 4) {
-#pragma unroll
-            for (int u = 0; u < 4; u++) {
-              const CR v = S + 4 * (j + u);
-              pair_grad<KIND, R, RSQ_MASK>(v[0] - ux, v[1] - uy, v[2] - uz, v[3], R(0), kr, ki, part);
-            }
+  for (int a = 0; a < NA; a++) { acc[a] = 0; comp[a] = 0; }
+  for (int g = g0; g < g1; g++) {
+    const int snode = t.p2p_src[g];
+    const int sb = t.src_begin[snode], ns = t.src_count[snode];
+    // target in the frame of the source leaf
+    const R ux = tx - R(t.center[3 * snode] - cx);
+    const R uy = ty - R(t.center[3 * snode + 1] - cy);
+    const R uz = tz - R(t.center[3 * snode + 2] - cz);
+    const bool self = snode == leaf;
+    if (split) {
+      R part[NA] = {0, 0, 0, 0};
+      if (act) {
+        if (self)
+          for (int j = ph; j < ns; j += k) {
+            const V v = srcv[sb + j];
+            pair_grad<KIND, R, RSQ_MASK>(v.x - ux, v.y - uy, v.z - uz, v.w, R(0), kr, ki, part);
           }
-          for (; j < m; j++) {
-            const CR v = S + 4 * j;
+        else
+          for (int j = ph; j < ns; j += k) {
+            const V v = srcv[sb + j];
+            pair_grad<KIND, R, RSQ_SEP>(v.x - ux, v.y - uy, v.z - uz, v.w, R(0), kr, ki, part);
+          }
+      }
+      fold_tile<R, NA>(acc, comp, part);
+      continue;
+    }
+    for (int s0 = 0; s0 < ns; s0 += TILE) {
+      const int m = min(TILE, ns - s0);
+      const CR S = src + 4 * size_t(sb + s0);
+      R part[NA] = {0, 0, 0, 0};
+      int j = 0;
+      if (self) {
+        for (; j + 4 <= m; j += 4) {
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const CR v = S + 4 * (j + u);
             pair_grad<KIND, R, RSQ_MASK>(v[0] - ux, v[1] - uy, v[2] - uz, v[3], R(0), kr, ki, part);
           }
-        } else {
-          for (; j + 4 <= m; j += 4) {
+        }
+        for (; j < m; j++) {
+          const CR v = S + 4 * j;
+          pair_grad<KIND, R, RSQ_MASK>(v[0] - ux, v[1] - uy, v[2] - uz, v[3], R(0), kr, ki, part);
+        }
+      } else {
+        for (; j + 4 <= m; j += 4) {
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-              const CR v = S + 4 * (j + u);
-              pair_grad<KIND, R, RSQ_SEP>(v[0] - ux, v[1] - uy, v[2] - uz, v[3], R(0), kr, ki, part);
-            }
-          }
-          for (; j < m; j++) {
-            const CR v = S + 4 * j;
+          for (int u = 0; u < 4; u++) {
+            const CR v = S + 4 * (j + u);
             pair_grad<KIND, R, RSQ_SEP>(v[0] - ux, v[1] - uy, v[2] - uz, v[3], R(0), kr, ki, part);
           }
         }
-        fold_tile<R, NA>(acc, comp, part);
+        for (; j < m; j++) {
+          const CR v = S + 4 * j;
+          pair_grad<KIND, R, RSQ_SEP>(v[0] - ux, v[1] - uy, v[2] - uz, v[3], R(0), kr, ki, part);
+        }
+      }
+      fold_tile<R, NA>(acc, comp, part);
+    }
+  }
+  double* o = t.trg_value + size_t(tb + t0) * 4;
+  if (split) {
+    split_reduce<R, NA>(acc, comp, nt, k, o + 4 * size_t(lane < nt ? lane : 0));
+  } else if (act) {
+#pragma unroll
+    for (int a = 0; a < NA; a++) o[4 * lane + a] += (double(acc[a]) - double(comp[a])) * FOUR_PI_INV;
+  }
+}
+
+// L2P of the real kernels (fmm_scale_invariant.h:198-230, fmm.h:188-218):
+// one wave per (leaf, tile of TILE targets), work as for k_p2p_smem.  The
+// sources, the leaf's equivalent surface points (per-level table, relative
+// to the centre) and its local equivalent densities, are the same for every
+// lane of a full tile and are read with scalar loads, four per unrolled
+// step; a tail tile of at most TILE / 2 targets splits them over lanes.
+template <int KIND, typename R>
+__global__ __launch_bounds__(64) void k_l2p_smem(DevTree t, Wavek wk, const int* __restrict__ work) {
+  constexpr int NA = 4;
+  typedef const double __attribute__((address_space(4))) * CD;
+  const int leaf = work[2 * blockIdx.x], t0 = TILE * work[2 * blockIdx.x + 1];
+  const int lane = threadIdx.x;
+  const double cx = t.center[3 * leaf], cy = t.center[3 * leaf + 1], cz = t.center[3 * leaf + 2];
+  const int tb = t.trg_begin[leaf], nt = min(TILE, t.trg_count[leaf] - t0), ns = t.nsurf;
+  const R kr = R(wk.re), ki = R(wk.im);
+  const size_t tab0 = size_t(t.level[leaf]) * ns;
+  const CD sp = (CD)(t.surf_tab) + tab0 * 4;
+  const CD eq = (CD)(t.dn) + size_t(leaf) * ns;
+  const bool split = 2 * nt <= TILE;
+  const int k = split ? TILE / nt : 1;
+  const int ti = split ? lane % nt : lane, ph = split ? lane / nt : 0;
+  const bool act = split ? ph < k : ti < nt;
+  R tx = 0, ty = 0, tz = 0;
+  if (ti < nt) {
+    const double* x = t.trg + 3 * size_t(tb + t0 + ti);
+    tx = R(x[0] - cx);
+    ty = R(x[1] - cy);
+    tz = R(x[2] - cz);
+  }
+  R acc[NA], comp[NA];
+#pragma unroll
+  for (int a = 0; a < NA; a++) { acc[a] = 0; comp[a] = 0; }
+  double* o = t.trg_value + size_t(tb + t0) * 4;
+  if (split) {
+    const double4* tab = reinterpret_cast<const double4*>(t.surf_tab) + tab0;
+    const double* eqv = t.dn + size_t(leaf) * ns;
+    for (int s0 = 0; s0 < ns; s0 += TILE * k) {
+      R part[NA] = {0, 0, 0, 0};
+      const int e = min(ns, s0 + TILE * k);
+      if (act)
+        for (int j = s0 + ph; j < e; j += k) {
+          const double4 v = tab[j];
+          pair_grad<KIND, R, RSQ_SEP>(R(v.x) - tx, R(v.y) - ty, R(v.z) - tz, R(eqv[j]), R(0), kr, ki, part);
+        }
+      fold_tile<R, NA>(acc, comp, part);
+    }
+    split_reduce<R, NA>(acc, comp, nt, k, o + 4 * size_t(lane < nt ? lane : 0));
+    return;
+  }
+  for (int s0 = 0; s0 < ns; s0 += TILE) {
+    const int m = min(TILE, ns - s0);
+    R part[NA] = {0, 0, 0, 0};
+    int j = 0;
+    for (; j + 4 <= m; j += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const CD v = sp + 4 * (s0 + j + u);
+        pair_grad<KIND, R, RSQ_SEP>(R(v[0]) - tx, R(v[1]) - ty, R(v[2]) - tz, R(eq[s0 + j + u]), R(0), kr, ki,
+                                    part);
       }
     }
-    if (act) {
-      double* o = t.trg_value + size_t(tb + ti) * 4;
-#pragma unroll
-      for (int a = 0; a < NA; a++) o[a] += (double(acc[a]) - double(comp[a])) * FOUR_PI_INV;
+    for (; j < m; j++) {
+      const CD v = sp + 4 * (s0 + j);
+      pair_grad<KIND, R, RSQ_SEP>(R(v[0]) - tx, R(v[1]) - ty, R(v[2]) - tz, R(eq[s0 + j]), R(0), kr, ki, part);
     }
+    fold_tile<R, NA>(acc, comp, part);
+  }
+  if (act) {
+#pragma unroll
+    for (int a = 0; a < NA; a++) o[4 * lane + a] += (double(acc[a]) - double(comp[a])) * FOUR_PI_INV;
   }
 }
 
@@ -462,27 +582,30 @@ __global__ void k_direct_reduce(const double* __restrict__ partial, int nsplit, 
 // ---------------------------------------------------------------- launchers
 
 template <int KIND, typename R>
-static void eval_targets_t(int mode, const DevTree& t, Wavek k, const int* work, int nwork, hipStream_t s) {
+static void eval_targets_t(int mode, const DevTree& t, Wavek k, const int* work, int nwork, const int* tiles,
+                           int ntiles, hipStream_t s) {
   if (nwork <= 0) return;
-  if (mode == MODE_P2P && KIND != HELMHOLTZ && t.src_pack && t.src_rel) {
+  const bool real = KIND != HELMHOLTZ && tiles && ntiles > 0;
+  if (mode == MODE_P2P && real && t.src_pack && t.src_rel) {
     if (t.nsrcs > 0)
       hipLaunchKernelGGL(k_pack_sources<R>, dim3((t.nsrcs + 255) / 256), dim3(256), 0, s, t.nsrcs, t.src_rel, t.q,
                          static_cast<typename P4<R>::t*>(t.src_pack));
-    hipLaunchKernelGGL((k_p2p_smem<KIND, R>), dim3(nwork), dim3(64), 0, s, t, k, work);
+    hipLaunchKernelGGL((k_p2p_smem<KIND, R>), dim3(ntiles), dim3(64), 0, s, t, k, tiles);
   } else if (mode == MODE_P2P) hipLaunchKernelGGL((k_eval_targets<KIND, R, MODE_P2P>), dim3(nwork), dim3(64), 0, s, t, k, work);
   else if (mode == MODE_M2P) hipLaunchKernelGGL((k_eval_targets<KIND, R, MODE_M2P>), dim3(nwork), dim3(64), 0, s, t, k, work);
+  else if (real && t.surf_tab) hipLaunchKernelGGL((k_l2p_smem<KIND, R>), dim3(ntiles), dim3(64), 0, s, t, k, tiles);
   else hipLaunchKernelGGL((k_eval_targets<KIND, R, MODE_L2P>), dim3(nwork), dim3(64), 0, s, t, k, work);
 }
 
 void launch_eval_targets(int kind, bool f32, int mode, const DevTree& t, Wavek k, const int* work, int nwork,
-                         hipStream_t s) {
+                         const int* tiles, int ntiles, hipStream_t s) {
   switch (kind) {
-    case LAPLACE: f32 ? eval_targets_t<LAPLACE, float>(mode, t, k, work, nwork, s)
-                      : eval_targets_t<LAPLACE, double>(mode, t, k, work, nwork, s); break;
-    case YUKAWA: f32 ? eval_targets_t<YUKAWA, float>(mode, t, k, work, nwork, s)
-                     : eval_targets_t<YUKAWA, double>(mode, t, k, work, nwork, s); break;
-    default: f32 ? eval_targets_t<HELMHOLTZ, float>(mode, t, k, work, nwork, s)
-                 : eval_targets_t<HELMHOLTZ, double>(mode, t, k, work, nwork, s); break;
+    case LAPLACE: f32 ? eval_targets_t<LAPLACE, float>(mode, t, k, work, nwork, tiles, ntiles, s)
+                      : eval_targets_t<LAPLACE, double>(mode, t, k, work, nwork, tiles, ntiles, s); break;
+    case YUKAWA: f32 ? eval_targets_t<YUKAWA, float>(mode, t, k, work, nwork, tiles, ntiles, s)
+                     : eval_targets_t<YUKAWA, double>(mode, t, k, work, nwork, tiles, ntiles, s); break;
+    default: f32 ? eval_targets_t<HELMHOLTZ, float>(mode, t, k, work, nwork, tiles, ntiles, s)
+                 : eval_targets_t<HELMHOLTZ, double>(mode, t, k, work, nwork, tiles, ntiles, s); break;
   }
 }
 
