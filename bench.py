@@ -4,7 +4,8 @@ One step = upward_pass + downward_pass of the whole tree with charges already
 resident in HBM (the FMM evaluation phase, every operator including the
 near-field P2P, which each upward_pass starts anew on a second stream; the
 host tree/list/precompute setup is reported separately).  The headline `value` is the mixed-precision path
-(FP32 P2P and M2L, FP64 pseudo-inverse solves and translations); the plain
+(FP32 P2P and M2L transforms, M2L Hadamard in split FP16 with FP32 accumulation, FP64 pseudo-inverse
+solves and translations); the plain
 FP64 path is timed beside it.  Errors are relative L2 against an FP64 direct
 sum on 2,000 seeded sample targets computed by the engine's direct-sum kernel.
 
@@ -252,7 +253,7 @@ def run_single_workload(a, name):
         "metric": "particles/s, %s FMM evaluation" % name, "value": m["value"], "unit": "particles/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": m["ms"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 P2P/M2L + f64 pinv solves" + (" (complex)" if kid == 1 else ""),
+        "dtype": "f32 P2P/M2L (f16-split Hadamard) + f64 pinv solves" + (" (complex)" if kid == 1 else ""),
         "data": "synthetic (seeded %s, charges %s)" % (dist, charges),
         "config": {"workload": "%s_N%d_p%d_ncrit%d" % (name, n, p, ncrit), "kernel": ["laplace", "helmholtz",
                    "modified_helmholtz"][kid], "wavek": wavek, "N": n, "p": p, "ncrit": ncrit,
@@ -302,7 +303,7 @@ def run_batched(a, nprob=64):
     return {
         "metric": "particles/s, laplace_batched FMM evaluation (aggregate)", "value": m["value"],
         "unit": "particles/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": m["ms"],
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 P2P/M2L + f64 pinv solves",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 P2P/M2L (f16-split Hadamard) + f64 pinv solves",
         "data": "synthetic (seeded Plummer, scale radius 1, truncated at r=10; charges 1/N)",
         "config": {"workload": "laplace_batched_%dproblems_p8_ncrit64" % nprob, "kernel": "laplace",
                    "problems": nprob, "N_total": int(ns.sum()), "N_min": int(ns.min()), "N_max": int(ns.max()),
@@ -420,7 +421,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 P2P/M2L + f64 pinv solves" if mixed else "f64",
+        "dtype": "f32 P2P/M2L (f16-split Hadamard) + f64 pinv solves" if mixed else "f64",
         "data": "synthetic (seeded uniform [0,1]^3, charges U[-1,1])",
         "config": {"workload": "laplace_fmm_N%d_p%d_ncrit%d_uniform" % (a.n, a.p, a.ncrit), "kernel": "laplace",
                    "N": a.n, "p": a.p, "ncrit": a.ncrit, "depth": prob["depth"], "nodes": int(flat["nnodes"]),
