@@ -15,6 +15,7 @@
 
 #include "../include/exafmm_t/build_list.h"
 #include "../include/exafmm_t/build_tree.h"
+#include "../include/exafmm_t/build_non_adaptive_tree.h"
 #include "../include/exafmm_t/helmholtz.h"
 #include "../include/exafmm_t/laplace.h"
 #include "../include/exafmm_t/modified_helmholtz.h"
@@ -79,6 +80,26 @@ Tree<T> setup(Bodies<T>& sources, Bodies<T>& targets, F& fmm) {
   init_rel_coord();
   set_colleagues<T>(tree.nodes);
   build_list<T>(tree.nodes, fmm);
+  fmm.M2L_setup(tree.nonleafs);
+  fmm.precompute();
+  return tree;
+}
+
+// setup of the reference's NON_ADAPTIVE build (python/exafmm.cpp:133-134 with
+// build_non_adaptive_tree.h; depth as the C++ tests set it,
+// tests/fmm_laplace.cpp:25-27): uniform tree to `depth`, lists with
+// target-only child lists (build_list.h:131-136), M2L_setup, precompute.
+template <typename T, typename F>
+Tree<T> setup_non_adaptive(Bodies<T>& sources, Bodies<T>& targets, F& fmm, int depth) {
+  if (sources.empty() || targets.empty()) throw std::runtime_error("setup: sources and targets must be non-empty");
+  if (depth < 0 || depth > 20) throw std::runtime_error("setup_non_adaptive: depth must be in [0, 20]");
+  Tree<T> tree;
+  get_bounds<T>(sources, targets, fmm.x0, fmm.r0);
+  fmm.depth = depth;
+  tree.nodes = build_uniform_tree<T>(sources, targets, tree.leafs, tree.nonleafs, fmm);
+  init_rel_coord();
+  set_colleagues<T>(tree.nodes);
+  build_list<T>(tree.nodes, fmm, true);
   fmm.M2L_setup(tree.nonleafs);
   fmm.precompute();
   return tree;
@@ -258,6 +279,9 @@ void bind_fmm(py::module& m, py::class_<F>& c) {
       .def("sync_equiv_to_host", [](F& f, Tree<T>& t) { f.sync_equiv_to_host(t.nodes); })
       .def("release_device", &F::release_device);
   m.def("setup", &setup<T, F>, "setup FMM: tree, 2:1 balance, lists, M2L setup and precomputation");
+  m.def("setup_non_adaptive", &setup_non_adaptive<T, F>,
+        "setup FMM on the non-adaptive (uniform) tree of the given depth: tree, lists, M2L setup, precomputation",
+        py::arg("sources"), py::arg("targets"), py::arg("fmm"), py::arg("depth"));
   m.def("evaluate", &evaluate<T, F>, "evaluate potential and gradient", py::arg("tree"), py::arg("fmm"),
         py::arg("verbose") = false);
 }

@@ -118,8 +118,12 @@ bool list_child_level(Node<T>* n, int nsurf) {
 
 }  // namespace detail
 
+// non_adaptive: child-level lists (P2P2, M2P) only for nodes with targets, as
+// the reference's NON_ADAPTIVE build does (build_list.h:131-136); in a
+// non-adaptive tree the empty (ghost) leaves above the leaf level would
+// otherwise see unrefined neighbours.
 template <typename T>
-void build_list(Nodes<T>& nodes, const FmmBase<T>& fmm) {
+void build_list(Nodes<T>& nodes, const FmmBase<T>& fmm, bool non_adaptive = EXAFMM_T_NON_ADAPTIVE) {
   init_rel_coord();
   const int npos = int(REL_COORD[M2L_Type].size());
   int unbalanced = 0;
@@ -132,7 +136,7 @@ void build_list(Nodes<T>& nodes, const FmmBase<T>& fmm) {
     n->M2L_list.assign(npos, nullptr);
     detail::list_parent_level(n, fmm.nsurf);
     detail::list_current_level(n);
-    if (!detail::list_child_level(n, fmm.nsurf)) unbalanced = 1;
+    if ((!non_adaptive || n->ntrgs > 0) && !detail::list_child_level(n, fmm.nsurf)) unbalanced = 1;
   }
   if (unbalanced) throw std::runtime_error("build_list: tree is not 2:1 balanced");
 }

@@ -167,6 +167,7 @@ struct TreeBuilder {
 
 }  // namespace detail
 
+#if !EXAFMM_T_NON_ADAPTIVE
 // Build the (unbalanced, or with `leafkeys` the balanced) tree.  The bodies are
 // reordered in place by octant path, as in the reference.
 template <typename T>
@@ -283,5 +284,6 @@ void balance_tree(Nodes<T>& nodes, Bodies<T>& sources, Bodies<T>& targets, NodeP
   // move, so the element storage did not change)
   fmm.depth = int(keys.size()) - 1;
 }
+#endif  // !EXAFMM_T_NON_ADAPTIVE
 
 }  // namespace exafmm_t

@@ -18,6 +18,16 @@
 #include <type_traits>
 #include <vector>
 
+// Tree variant (reference: -DNON_ADAPTIVE selects build_non_adaptive_tree.h
+// and the target-only child lists of build_list.h:131-136).  The non-adaptive
+// builder is always available as build_uniform_tree; the macro only decides
+// which builder the reference's name build_tree denotes.
+#if defined(NON_ADAPTIVE) && NON_ADAPTIVE
+#define EXAFMM_T_NON_ADAPTIVE 1
+#else
+#define EXAFMM_T_NON_ADAPTIVE 0
+#endif
+
 namespace exafmm_t {
 
 // Host arithmetic is always FP64; the device precision mode (FP64 or mixed)

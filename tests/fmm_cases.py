@@ -38,7 +38,9 @@ def points(n, dist, rng):
     raise ValueError(dist)
 
 
-def make_problem(kernel, n, p, ncrit=64, dist="cube", seed=0, wavek=None, precision=0):
+def make_problem(kernel, n, p, ncrit=64, dist="cube", seed=0, wavek=None, precision=0, depth=None):
+    """depth=None: adaptive tree (setup); depth=d: the non-adaptive tree of that
+    depth (setup_non_adaptive, the reference's NON_ADAPTIVE build)."""
     mod, cls, kid, wk_default = KERNELS[kernel]
     wk = wk_default if wavek is None else wavek
     rng = np.random.default_rng(seed)
@@ -52,7 +54,7 @@ def make_problem(kernel, n, p, ncrit=64, dist="cube", seed=0, wavek=None, precis
     fname = os.path.join(CACHE, "%s_p%d_k%s_%d.dat" % (kernel, p, str(wk).replace(" ", ""), seed))
     fmm = Cls(p, ncrit, filename=fname) if wk is None else Cls(p, ncrit, wk, filename=fname)
     fmm.precision = precision
-    tree = mod.setup(src, trg, fmm)
+    tree = mod.setup(src, trg, fmm) if depth is None else mod.setup_non_adaptive(src, trg, fmm, depth)
     return dict(mod=mod, fmm=fmm, tree=tree, kid=kid, wavek=0.0 if wk is None else wk, x=x, q=q, p=p,
                 flat=fmm.flat_tree(tree), ops=fmm.operators(), qs=fmm.sorted_charges(tree))
 

@@ -137,6 +137,27 @@ def test_mixed_precision(require_gpu, kernel, p, dist, n, ncrit):
     assert emx < max(2 * e64, 1e-5)
 
 
+@pytest.mark.parametrize("kernel,p,dist,depth", [("laplace", 6, "clusters", 4), ("modified_helmholtz", 5, "sphere", 4),
+                                                ("helmholtz", 5, "cube", 3)])
+def test_non_adaptive_tree(require_gpu, kernel, p, dist, depth):
+    """Non-adaptive tree (the reference's NON_ADAPTIVE build: uniform to
+    `depth`, ghost empty nodes, target-only child lists): FP64 full passes
+    equal the oracle to 1e-10, the mixed path stays within 1e-5 of them."""
+    pr = make_problem(kernel, 12000, p, dist=dist, seed=8, depth=depth)
+    print("non-adaptive case", kernel, p, dist, depth, "nodes", pr["flat"]["nnodes"], flush=True)
+    orc = O.Oracle(pr["kid"], p, pr["flat"], pr["ops"], pr["wavek"])
+    orc.set_charges(pr["qs"])
+    ref = orc.evaluate()
+    for prec, tol in ((abi.FP64, FP64_TOL), (abi.MIXED, 1e-5)):
+        eng = _engine(pr, prec)
+        eng.upward_pass()
+        eng.downward_pass()
+        v = eng.values()
+        print("precision", prec, "phi %.2e grad %.2e" % (rel_err(v[:, 0], ref[:, 0]), rel_err(v[:, 1:], ref[:, 1:])))
+        assert rel_err(v[:, 0], ref[:, 0]) < tol
+        assert rel_err(v[:, 1:], ref[:, 1:]) < (tol if prec == abi.FP64 else 1e-4)
+
+
 @pytest.mark.parametrize("kernel,p,dist", [("laplace", 10, "clusters"), ("helmholtz", 5, "cube")])
 def test_mixed_hadamard_fp16_vs_fp32(require_gpu, monkeypatch, kernel, p, dist):
     """The default mixed Hadamard (split FP16 on the f16 matrix cores) and the

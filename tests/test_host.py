@@ -60,6 +60,41 @@ def test_tree_leaf_capacity_and_balance(problem):
     assert np.all(np.diff(lev) >= 0)
 
 
+@pytest.fixture(scope="module", params=[("laplace", "cube", 3), ("laplace", "clusters", 4),
+                                        ("helmholtz", "sphere", 3)])
+def uniform_problem(request):
+    kernel, dist, depth = request.param
+    return make_problem(kernel, 5000, 4, dist=dist, seed=4, depth=depth)
+
+
+def test_non_adaptive_tree_structure(uniform_problem):
+    """Non-adaptive tree (reference build_non_adaptive_tree.h:95-194): every
+    body once, all bodies in leaves at `depth`, an empty node is a leaf at any
+    level and never split, a node with bodies above `depth` is split."""
+    f = uniform_problem["flat"]
+    depth = uniform_problem["fmm"].depth
+    n = len(uniform_problem["x"])
+    assert f["depth"] == depth
+    assert sorted(f["src_ibody"].tolist()) == list(range(n))
+    assert sorted(f["trg_ibody"].tolist()) == list(range(n))
+    leaf = f["child0"] < 0
+    lev = f["level"]
+    has = (f["src_count"] > 0) | (f["trg_count"] > 0)
+    assert np.all(lev[leaf & has] == depth)
+    c0 = f["child0"]
+    sub = f["src_count"].astype(np.int64) + f["trg_count"]
+    for nd in sorted(np.nonzero(~leaf)[0], key=lambda i: -lev[i]):
+        sub[nd] = sub[c0[nd]:c0[nd] + 8].sum()
+    assert np.all(sub[~leaf] > 0)              # split nodes hold bodies
+    assert np.all(lev[~leaf] < depth)
+    assert np.all(sub[leaf & (lev < depth)] == 0)  # leaves above depth are ghosts
+    assert np.array_equal(f["src_coord"].reshape(-1, 3), uniform_problem["x"][f["src_ibody"]])
+
+
+def test_non_adaptive_lists_complete(uniform_problem):
+    test_lists_complete_and_disjoint(uniform_problem)
+
+
 def test_lists_complete_and_disjoint(problem):
     """Counting FMM (reference tests/list.cpp:44 with test.h DummyFmm): every
     target leaf receives every source exactly once through P2P, M2P, P2L (at

@@ -184,6 +184,19 @@ def test_fft_m2l_equals_dense_m2l(kid, wk):
                                           ("helmholtz", 5, 1e-4)])
 def test_oracle_fmm_against_direct(kernel, p, tol):
     pr = make_problem(kernel, 3000, p, ncrit=32, dist="cube", seed=1)
+    _fmm_vs_direct(pr, p, tol)
+
+
+@pytest.mark.parametrize("kernel,p,dist,depth,tol", [("laplace", 6, "cube", 3, 2e-5), ("laplace", 6, "clusters", 4, 2e-5),
+                                                     ("modified_helmholtz", 5, "sphere", 3, 1e-4)])
+def test_oracle_fmm_non_adaptive_against_direct(kernel, p, dist, depth, tol):
+    """The same evaluation on the non-adaptive tree (reference NON_ADAPTIVE
+    build: uniform to `depth`, ghost empty nodes, target-only child lists)."""
+    pr = make_problem(kernel, 3000, p, dist=dist, seed=6, depth=depth)
+    _fmm_vs_direct(pr, p, tol)
+
+
+def _fmm_vs_direct(pr, p, tol):
     orc = O.Oracle(pr["kid"], p, pr["flat"], pr["ops"], pr["wavek"])
     orc.set_charges(pr["qs"])
     v = orc.evaluate()
