@@ -71,6 +71,7 @@ struct xfmm_context {
   // (transforms + Hadamard), and the surface evaluations P2M/L2P/M2P/P2L
   bool f32_p2p = false, f32_m2l = false, f32_surf = false, f32_m2l_fft = false, f32_far = false;
   bool f16_had = false;  // split-FP16 Hadamard (needs f32_m2l)
+  bool f32_gemm = false;  // FP32 MFMA translation GEMMs (FLOAT mode: float-truncated pseudo-inverses)
   // dn and trg_value are zero (cleared by the last upward pass or clear_values
   // and not written since), so the downward pass need not clear them itself
   bool down_clean = false;
@@ -303,9 +304,9 @@ void op_P2M(xfmm_context* c) {
     const int b = c->leaf_level_off[l], e = c->leaf_level_off[l + 1];
     if (e <= b) continue;
     const int L = c->op_level(l);
-    launch_gemm_rows(c->cplx, e - b, c->nsurf, c->nsurf, c->check.ptr, c->leaf_slot_rows.ptr + b,
+    launch_gemm_rows(c->cplx, c->f32_gemm, e - b, c->nsurf, c->nsurf, c->check.ptr, c->leaf_slot_rows.ptr + b,
                      c->uc2e_uT.ptr + L * ns2, c->tmp.ptr, c->leaf_slot_rows.ptr + b, 1.0, 0, c->stream);
-    launch_gemm_rows(c->cplx, e - b, c->nsurf, c->nsurf, c->tmp.ptr, c->leaf_slot_rows.ptr + b,
+    launch_gemm_rows(c->cplx, c->f32_gemm, e - b, c->nsurf, c->nsurf, c->tmp.ptr, c->leaf_slot_rows.ptr + b,
                      c->uc2e_vT.ptr + L * ns2, c->up.ptr, c->leaf_node_rows.ptr + b, c->leaf_scale(l), 0,
                      c->stream);
   }
@@ -318,7 +319,7 @@ void op_M2M(xfmm_context* c) {
     if (e <= b) continue;
     const int L = c->op_level(l);
     // parent row = [child_0 .. child_7] (contiguous, K = 8 nsurf) x M2Mcat
-    launch_gemm_rows(c->cplx, e - b, c->nsurf, 8 * c->nsurf, c->up.ptr, c->nl_child_rows.ptr + b,
+    launch_gemm_rows(c->cplx, c->f32_gemm, e - b, c->nsurf, 8 * c->nsurf, c->up.ptr, c->nl_child_rows.ptr + b,
                      c->m2mcat.ptr + L * blk, c->up.ptr, c->nl_rows.ptr + b, 1.0, 1, c->stream,
                      c->gemm_scratch.ptr, c->gemm_scratch.n);
   }
@@ -330,7 +331,8 @@ void op_L2L(xfmm_context* c) {
     const int b = c->nl_level_off[l], e = c->nl_level_off[l + 1];
     if (e <= b) continue;
     const int L = c->op_level(l);
-    launch_gemm_rows(c->cplx, e - b, 8 * c->nsurf, c->nsurf, c->dn.ptr, c->nl_rows.ptr + b, c->l2lcat.ptr + L * blk,
+    launch_gemm_rows(c->cplx, c->f32_gemm, e - b, 8 * c->nsurf, c->nsurf, c->dn.ptr, c->nl_rows.ptr + b,
+                     c->l2lcat.ptr + L * blk,
                      c->dn.ptr, c->nl_child_rows.ptr + b, 1.0, 1, c->stream, c->gemm_scratch.ptr,
                      c->gemm_scratch.n);
   }
@@ -342,9 +344,9 @@ void op_L2P(xfmm_context* c) {
     const int b = c->leaf_level_off[l], e = c->leaf_level_off[l + 1];
     if (e <= b) continue;
     const int L = c->op_level(l);
-    launch_gemm_rows(c->cplx, e - b, c->nsurf, c->nsurf, c->dn.ptr, c->leaf_node_rows.ptr + b,
+    launch_gemm_rows(c->cplx, c->f32_gemm, e - b, c->nsurf, c->nsurf, c->dn.ptr, c->leaf_node_rows.ptr + b,
                      c->dc2e_uT.ptr + L * ns2, c->tmp.ptr, c->leaf_slot_rows.ptr + b, 1.0, 0, c->stream);
-    launch_gemm_rows(c->cplx, e - b, c->nsurf, c->nsurf, c->tmp.ptr, c->leaf_slot_rows.ptr + b,
+    launch_gemm_rows(c->cplx, c->f32_gemm, e - b, c->nsurf, c->nsurf, c->tmp.ptr, c->leaf_slot_rows.ptr + b,
                      c->dc2e_vT.ptr + L * ns2, c->dn.ptr, c->leaf_node_rows.ptr + b, c->leaf_scale(l), 0,
                      c->stream);
   }
@@ -444,7 +446,7 @@ int xfmm_create(int kernel, int p, double wavek_re, double wavek_im, int precisi
   if (kernel < XFMM_LAPLACE || kernel > XFMM_MODIFIED_HELMHOLTZ)
     return fail(nullptr, XFMM_ERR_ARG, "xfmm_create: unknown kernel");
   if (p < 2 || p > 16) return fail(nullptr, XFMM_ERR_ARG, "xfmm_create: expansion order p must be in [2, 16]");
-  if (precision != XFMM_FP64 && precision != XFMM_MIXED)
+  if (precision != XFMM_FP64 && precision != XFMM_MIXED && precision != XFMM_FLOAT)
     return fail(nullptr, XFMM_ERR_ARG, "xfmm_create: unknown precision mode");
   xfmm_context* c = new (std::nothrow) xfmm_context;
   if (!c) return fail(nullptr, XFMM_ERR_NOMEM, "xfmm_create: host allocation failed");
@@ -457,8 +459,8 @@ int xfmm_create(int kernel, int p, double wavek_re, double wavek_im, int precisi
     // per FP32 value, FP32 accumulation).  XFMM_MIXED_STAGES overrides it with
     // a comma list of p2p, m2l (FP32 spectra + Hadamard), had16 (split-FP16
     // Hadamard, with m2l), fft (FP32 transform arithmetic, else FP64), far
-    // (FP32 L2P / M2P evaluation) and surf (FP32 check potentials of P2M /
-    // P2L).  "far" is off by default: the FP32 M2L
+    // (FP32 L2P / M2P evaluation), surf (FP32 check potentials of P2M /
+    // P2L) and gemm (FP32 translation GEMMs).  "far" is off by default: the FP32 M2L
     // noise, amplified by the downward pseudo-inverse, gives the local
     // equivalent densities large components that cancel only when the L2P
     // sum is done in FP64.  "surf" loses all accuracy through the
@@ -471,6 +473,9 @@ int xfmm_create(int kernel, int p, double wavek_re, double wavek_im, int precisi
     c->f32_far = st.find("far") != std::string::npos;
     c->f32_m2l_fft = c->f32_m2l && st.find("fft") != std::string::npos;
     c->f16_had = c->f32_m2l && st.find("had16") != std::string::npos;
+    c->f32_gemm = st.find("gemm") != std::string::npos;
+  } else if (precision == XFMM_FLOAT) {
+    c->f32_p2p = c->f32_m2l = c->f32_m2l_fft = c->f16_had = c->f32_surf = c->f32_far = c->f32_gemm = true;
   }
   c->device = device;
   c->wk_re = wavek_re;
@@ -1200,7 +1205,7 @@ int xfmm_direct(int kernel, int precision, double wavek_re, double wavek_im, con
     const int tblocks = (nt + 63) / 64;
     int nsplit = std::max(1, std::min(2048 / tblocks + 1, (ns + 255) / 256));
     scratch.alloc(size_t(nsplit) * nt * na);
-    launch_direct(kernel, precision == XFMM_MIXED, vw == 2, Wavek{wavek_re, wavek_im}, dsrc.ptr, dq.ptr, ns, dtrg.ptr, nt,
+    launch_direct(kernel, precision != XFMM_FP64, vw == 2, Wavek{wavek_re, wavek_im}, dsrc.ptr, dq.ptr, ns, dtrg.ptr, nt,
                   dout.ptr, gradient, scratch.ptr, nsplit, s);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(trg_value, dout.ptr, size_t(nt) * na * sizeof(double), hipMemcpyDeviceToHost, s));

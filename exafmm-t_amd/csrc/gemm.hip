@@ -14,6 +14,8 @@
 // has too few tiles to fill the chip).  Complex operands (Helmholtz) use a
 // VALU kernel: 256 threads, 64 x 64 tile, 4 x 4 complex outputs per thread,
 // K staged through LDS in steps of 8.
+#include <stdexcept>
+
 #include "kernels.h"
 
 namespace xfmm {
@@ -95,15 +97,37 @@ __global__ __launch_bounds__(256) void k_gemm_rows_cplx(int m, int n, int k, con
 // XCD (workgroup w goes to XCD w % 8), which then reads that band of A once
 // into its L2.  Optional split-K (grid.y): each split writes its partial
 // product to `partial` and k_gemm_reduce combines them in a fixed order.
-typedef double d4 __attribute__((ext_vector_type(4)));
 constexpr int GM_BK = 16;
+
+// Compute type of the MFMA path: FP64 (v_mfma_f64_16x16x4_f64, result
+// register q of lane group g = row g + 4q) or FP32 (v_mfma_f32_16x16x4_f32,
+// row 4g + q) for operators that tolerate it; operands are FP64 in memory
+// either way and converted while staged, results are stored as FP64.
+template <typename CT>
+struct GemmMfma;
+template <>
+struct GemmMfma<double> {
+  typedef double v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int g, int q) { return g + 4 * q; }
+};
+template <>
+struct GemmMfma<float> {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int g, int q) { return 4 * g + q; }
+};
 
 // Without `partial` the kernel writes C itself: alpha scales the rows of A as
 // they are staged (exact: alpha is a power of two) and BETA adds C in the
 // epilogue.  BETA = false keeps the epilogue a plain store, so the
 // accumulators never leave the AGPRs before it (BETA = true costs occupancy
 // and is used only when no split-K scratch is available).
-template <int BM, int BN, bool BETA>
+template <int BM, int BN, bool BETA, typename CT>
 __global__ __launch_bounds__(256) void k_gemm_rows_mfma64(int m, int n, int k, int kper, int ncol,
                                                          const double* __restrict__ A, const long* __restrict__ arow,
                                                          const double* __restrict__ B, double* __restrict__ C,
@@ -113,8 +137,9 @@ __global__ __launch_bounds__(256) void k_gemm_rows_mfma64(int m, int n, int k, i
   constexpr int NQA = BM * BK / 256, NQB = BK * BN / 256;  // loader elements per thread
   constexpr int ARS = 256 / BK, BKS = 256 / BN;            // loader row / k strides
   static_assert(BM % 32 == 0 && BN % 32 == 0 && BN <= 256 && NQA >= 1 && NQB >= 1, "tile shape");
-  __shared__ double As[BK][BM + 2];
-  __shared__ double Bs[BK][BN + 2];
+  typedef typename GemmMfma<CT>::v4 d4;
+  __shared__ CT As[BK][BM + 2];
+  __shared__ CT Bs[BK][BN + 2];
   __shared__ long arow_s[BM];
   // XCD ordering when there are at least 8 row bands (grid padded to a
   // multiple of 8 bands), plain row-major tile order otherwise
@@ -166,15 +191,15 @@ __global__ __launch_bounds__(256) void k_gemm_rows_mfma64(int m, int n, int k, i
     // (the scale is applied here, when the loads have arrived anyway, so the
     // prefetch of the next stage is not waited for before the MFMAs)
 #pragma unroll
-    for (int q = 0; q < NQA; q++) As[akk][ar0 + ARS * q] = scale * ra[q];
+    for (int q = 0; q < NQA; q++) As[akk][ar0 + ARS * q] = CT(scale * ra[q]);
 #pragma unroll
-    for (int q = 0; q < NQB; q++) Bs[bkk + BKS * q][bc] = rb[q];
+    for (int q = 0; q < NQB; q++) Bs[bkk + BKS * q][bc] = CT(rb[q]);
     __syncthreads();
     if (k0 + BK < kend) fetch(k0 + BK);
 #pragma unroll
     for (int s = 0; s < BK / 4; s++) {
       const int kk = 4 * s + g;
-      double av[TA], bv[TB];
+      CT av[TA], bv[TB];
 #pragma unroll
       for (int t = 0; t < TA; t++) av[t] = As[kk][wr * WM + t * 16 + j];
 #pragma unroll
@@ -182,24 +207,26 @@ __global__ __launch_bounds__(256) void k_gemm_rows_mfma64(int m, int n, int k, i
 #pragma unroll
       for (int a = 0; a < TA; a++)
 #pragma unroll
-        for (int b = 0; b < TB; b++) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+        for (int b = 0; b < TB; b++) acc[a][b] = GemmMfma<CT>::mma(av[a], bv[b], acc[a][b]);
     }
     __syncthreads();
   }
-  // result register q of lane (g, j): row g + 4q, column j of the 16 x 16 tile
+  // result register q of lane (g, j): row GemmMfma::row(g, q), column j of
+  // the 16 x 16 tile
 #pragma unroll
   for (int a = 0; a < TA; a++)
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      const int r = row0 + wr * WM + a * 16 + g + 4 * q;
+      const int r = row0 + wr * WM + a * 16 + GemmMfma<CT>::row(g, q);
       if (r >= m) continue;
 #pragma unroll
       for (int b = 0; b < TB; b++) {
         const int cc = col0 + wc * WN + b * 16 + j;
         if (cc >= n) continue;
-        if (!direct) partial[(size_t(blockIdx.y) * m + r) * n + cc] = acc[a][b][q];
-        else if (BETA) C[crow[r] + cc] += acc[a][b][q];
-        else C[crow[r] + cc] = acc[a][b][q];
+        const double v = double(acc[a][b][q]);
+        if (!direct) partial[(size_t(blockIdx.y) * m + r) * n + cc] = v;
+        else if (BETA) C[crow[r] + cc] += v;
+        else C[crow[r] + cc] = v;
       }
       // one accumulator tile at a time through the VGPRs in the BETA epilogue
       if (BETA) __builtin_amdgcn_sched_barrier(0);
@@ -243,7 +270,7 @@ size_t gemm_scratch_doubles(int m, int n, int k) {
   return split > 1 ? size_t(split) * m * n : 0;
 }
 
-template <int BM, int BN, bool BETA>
+template <int BM, int BN, bool BETA, typename CT>
 static void launch_mfma(int m, int n, int k, int split, const double* A, const long* arow, const double* B, double* C,
                         const long* crow, double alpha, double* partial, hipStream_t s) {
   int kper = (k + split - 1) / split;
@@ -252,44 +279,53 @@ static void launch_mfma(int m, int n, int k, int split, const double* A, const l
   const int nband = (m + BM - 1) / BM;
   const int nb = nband >= 8 ? (nband + 7) / 8 * 8 : nband;
   dim3 grid(unsigned(nb * ncol), split);
-  hipLaunchKernelGGL((k_gemm_rows_mfma64<BM, BN, BETA>), grid, dim3(256), 0, s, m, n, k, kper, ncol, A, arow, B, C,
-                     crow, alpha, partial);
+  hipLaunchKernelGGL((k_gemm_rows_mfma64<BM, BN, BETA, CT>), grid, dim3(256), 0, s, m, n, k, kper, ncol, A, arow,
+                     B, C, crow, alpha, partial);
 }
 
-template <int BM, int BN>
+template <int BM, int BN, typename CT>
 static void launch_tile(int m, int n, int k, int split, bool part, const double* A, const long* arow, const double* B,
                         double* C, const long* crow, double alpha, int beta, double* scratch, hipStream_t s) {
-  if (part) launch_mfma<BM, BN, false>(m, n, k, split, A, arow, B, C, crow, alpha, scratch, s);
-  else if (beta) launch_mfma<BM, BN, true>(m, n, k, 1, A, arow, B, C, crow, alpha, nullptr, s);
-  else launch_mfma<BM, BN, false>(m, n, k, 1, A, arow, B, C, crow, alpha, nullptr, s);
-}
-// (the 128 x 128 += epilogue is never launched: it would cost occupancy)
-template <>
-void launch_tile<128, 128>(int m, int n, int k, int split, bool part, const double* A, const long* arow,
-                           const double* B, double* C, const long* crow, double alpha, int beta, double* scratch,
-                           hipStream_t s) {
-  launch_mfma<128, 128, false>(m, n, k, split, A, arow, B, C, crow, alpha, part ? scratch : nullptr, s);
+  // (the 128 x 128 += epilogue is never built: it would cost occupancy;
+  // launch_real picks 128 x 128 only for split-K or plain stores)
+  if (part) {
+    launch_mfma<BM, BN, false, CT>(m, n, k, split, A, arow, B, C, crow, alpha, scratch, s);
+  } else if constexpr (BM != 128) {
+    if (beta) launch_mfma<BM, BN, true, CT>(m, n, k, 1, A, arow, B, C, crow, alpha, nullptr, s);
+    else launch_mfma<BM, BN, false, CT>(m, n, k, 1, A, arow, B, C, crow, alpha, nullptr, s);
+  } else {
+    if (beta) throw std::logic_error("launch_tile: accumulating 128 x 128 tile");
+    launch_mfma<BM, BN, false, CT>(m, n, k, 1, A, arow, B, C, crow, alpha, nullptr, s);
+  }
 }
 
-void launch_gemm_rows(bool cplx, int m, int n, int k, const double* A, const long* arow, const double* B,
+template <typename CT>
+static void launch_real(int m, int n, int k, const double* A, const long* arow, const double* B, double* C,
+                        const long* crow, double alpha, int beta, hipStream_t s, double* scratch,
+                        size_t scratch_doubles) {
+  // an accumulating product without split-K uses the 64 x 64 tile, whose
+  // += epilogue keeps full occupancy
+  int split = gemm_split(m, n, k);
+  const bool part = scratch && split > 1 && size_t(split) * m * n <= scratch_doubles;
+  if (!part) split = 1;
+  if (gemm_tile(m).bm == 128 && (part || !beta))
+    launch_tile<128, 128, CT>(m, n, k, split, part, A, arow, B, C, crow, alpha, beta, scratch, s);
+  else
+    launch_tile<64, 64, CT>(m, n, k, split, part, A, arow, B, C, crow, alpha, beta, scratch, s);
+  if (part) {
+    const size_t tot = size_t(m) * n;
+    hipLaunchKernelGGL(k_gemm_reduce, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, s, m, n, split, scratch, C,
+                       crow, alpha, beta);
+  }
+}
+
+void launch_gemm_rows(bool cplx, bool f32, int m, int n, int k, const double* A, const long* arow, const double* B,
                       double* C, const long* crow, double alpha, int beta, hipStream_t s, double* scratch,
                       size_t scratch_doubles) {
   if (m <= 0 || n <= 0) return;
   if (!cplx) {
-    // an accumulating product without split-K uses the 64 x 64 tile, whose
-    // += epilogue keeps full occupancy
-    int split = gemm_split(m, n, k);
-    const bool part = scratch && split > 1 && size_t(split) * m * n <= scratch_doubles;
-    if (!part) split = 1;
-    if (gemm_tile(m).bm == 128 && (part || !beta))
-      launch_tile<128, 128>(m, n, k, split, part, A, arow, B, C, crow, alpha, beta, scratch, s);
-    else
-      launch_tile<64, 64>(m, n, k, split, part, A, arow, B, C, crow, alpha, beta, scratch, s);
-    if (part) {
-      const size_t tot = size_t(m) * n;
-      hipLaunchKernelGGL(k_gemm_reduce, dim3(unsigned((tot + 255) / 256)), dim3(256), 0, s, m, n, split, scratch, C,
-                         crow, alpha, beta);
-    }
+    if (f32) launch_real<float>(m, n, k, A, arow, B, C, crow, alpha, beta, s, scratch, scratch_doubles);
+    else launch_real<double>(m, n, k, A, arow, B, C, crow, alpha, beta, s, scratch, scratch_doubles);
   } else {
     constexpr int BM = 64, BN = 64;
     dim3 grid((n + BN - 1) / BN, (m + BM - 1) / BM);

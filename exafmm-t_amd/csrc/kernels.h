@@ -64,10 +64,12 @@ void launch_direct(int kind, bool mixed, bool cplx, Wavek k, const double* src, 
 // C[crow[i]] (beta=0: =, beta=1: +=) alpha * A[arow[i]] x B for i < m.
 // A rows have K values, B is K x N row-major, C rows have N values (kind T).
 // arow/crow are row offsets in values (not rows); alpha is a power of two.
-// Real GEMMs run on FP64 MFMA; when the output has few tiles the K range is
-// split over workgroups, partial products go to `scratch` (at least
-// gemm_scratch_doubles(m, n, k) doubles) and are reduced in a fixed order.
-void launch_gemm_rows(bool cplx, int m, int n, int k, const double* A, const long* arow, const double* B,
+// Real GEMMs run on FP64 MFMA, or with f32 on FP32 MFMA (operands rounded to
+// FP32 while staged, FP32 accumulation, FP64 results); when the output has few
+// tiles the K range is split over workgroups, partial products go to
+// `scratch` (at least gemm_scratch_doubles(m, n, k) doubles) and are reduced
+// in a fixed order.  Complex GEMMs are FP64 (f32 ignored).
+void launch_gemm_rows(bool cplx, bool f32, int m, int n, int k, const double* A, const long* arow, const double* B,
                       double* C, const long* crow, double alpha, int beta, hipStream_t s,
                       double* scratch = nullptr, size_t scratch_doubles = 0);
 size_t gemm_scratch_doubles(int m, int n, int k);

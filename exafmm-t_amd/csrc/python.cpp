@@ -271,7 +271,9 @@ void bind_fmm(py::module& m, py::class_<F>& c) {
       .def_readonly("r0", &F::r0)
       .def_property_readonly("x0", [](const F& f) { return std::vector<double>{f.x0[0], f.x0[1], f.x0[2]}; })
       .def_readwrite("filename", &F::filename)
-      .def_readwrite("precision", &F::precision, "0: FP64 device path, 1: mixed (FP32 P2P and M2L)")
+      .def_readwrite("precision", &F::precision,
+                     "0: FP64 device path, 1: mixed (FP32 P2P and M2L), 2: FLOAT (all evaluation stages FP32, "
+                     "pseudo-inverses truncated at 4e-8; set before setup)")
       .def_readwrite("device", &F::device)
       .def("flat_tree", [](F& f, Tree<T>& t) { return flat_tree<T, F>(f, t); })
       .def("operators", [](F& f) { return operators<T, F>(f); })

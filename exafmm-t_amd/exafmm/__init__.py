@@ -25,5 +25,6 @@ for _name in ("laplace", "helmholtz", "modified_helmholtz"):
 
 FP64 = 0
 MIXED = 1
+FLOAT = 2
 
-__all__ = ["laplace", "helmholtz", "modified_helmholtz", "FP64", "MIXED"]
+__all__ = ["laplace", "helmholtz", "modified_helmholtz", "FP64", "MIXED", "FLOAT"]

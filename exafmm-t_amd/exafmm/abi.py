@@ -11,7 +11,7 @@ import os
 import numpy as np
 
 LAPLACE, HELMHOLTZ, MODIFIED_HELMHOLTZ = 0, 1, 2
-FP64, MIXED = 0, 1
+FP64, MIXED, FLOAT = 0, 1, 2
 LIB_PATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lib",
                         "libexafmm_t_amd.so")
 TIMING_NAMES = ["P2M", "M2M", "P2L", "M2P", "P2P", "M2L", "L2L", "L2P", "FFT", "HAD", "IFFT"]

@@ -37,7 +37,7 @@ class FmmBase {
   bool is_precomputed = false;
   bool is_real = std::is_same<T, real_t>::value;
   std::string filename;
-  int precision = XFMM_FP64;  // XFMM_FP64 or XFMM_MIXED (device arithmetic of P2P and M2L)
+  int precision = XFMM_FP64;  // XFMM_FP64, XFMM_MIXED or XFMM_FLOAT (device arithmetic of the stages)
   int device = 0;             // HIP device ordinal
 
   // Precomputed operators, FP64, indexed by level (a single level for the
@@ -104,7 +104,7 @@ class FmmBase {
   // Pseudo-inverse factors of the check-to-equivalent matrix at `level`
   // (fmm_scale_invariant.h:446-514, fmm.h:451-521): A = U S V^H,
   // UC2E_U = U^H, UC2E_V = V S^+, DC2E_U = V^T (= VH for real), DC2E_V = conj(U) S^+,
-  // singular values below 4 EPS max(S) dropped.
+  // singular values below 4 eps max(S) dropped (eps = pinv_eps()).
   void precompute_check2equiv(int level) {
     real_t c[3] = {0, 0, 0};
     RealVec up_check = surface(p, r0, level, c, 2.95);
@@ -116,7 +116,8 @@ class FmmBase {
     svd(nsurf, A, U, S, VH);
     const real_t smax = S.empty() ? 0 : S[0];
     RealVec Sinv(nsurf);
-    for (int i = 0; i < nsurf; i++) Sinv[i] = S[i] > EPS * smax * 4 ? 1.0 / S[i] : 0.0;
+    const real_t eps = pinv_eps();
+    for (int i = 0; i < nsurf; i++) Sinv[i] = S[i] > eps * smax * 4 ? 1.0 / S[i] : 0.0;
     std::vector<T> UH = conjugate_transpose(U, nsurf, nsurf);
     std::vector<T> V = conjugate_transpose(VH, nsurf, nsurf);
     std::vector<T>& uu = matrix_UC2E_U[level];
@@ -247,18 +248,26 @@ class FmmBase {
     return bool(f);
   }
 
+  // Truncation of the check-to-equivalent pseudo-inverses: EPS (1e-16), or
+  // 1e-8 for the FLOAT mode, the EPS of the reference's -DFLOAT build
+  // (exafmm_t.h:23).
+  real_t pinv_eps() const { return precision == XFMM_FLOAT ? real_t(1e-8) : EPS; }
+
   // precompute(): needs r0 (get_bounds) and, for level-dependent kernels, depth.
+  // The cache file holds operators of the default truncation only (the FLOAT
+  // mode's operators are computed, not cached).
   void precompute() {
     init_rel_coord();
     initialize_matrix();
-    is_precomputed = load_matrix();
+    const bool cacheable = pinv_eps() == EPS;
+    is_precomputed = cacheable && load_matrix();
     if (!is_precomputed) {
       for (int l = 0; l < op_levels(); l++) precompute_check2equiv(l);
       for (int l = 0; l < op_levels(); l++) precompute_M2M(l);
       if (scale_invariant()) precompute_M2L(0, 0);
       else
         for (int l = 1; l <= depth; l++) precompute_M2L(l - 1, l);
-      save_matrix();
+      if (cacheable) save_matrix();
       is_precomputed = true;
     }
     ops_uploaded = false;

@@ -29,7 +29,7 @@ extern "C" {
 #endif
 
 enum { XFMM_LAPLACE = 0, XFMM_HELMHOLTZ = 1, XFMM_MODIFIED_HELMHOLTZ = 2 };
-enum { XFMM_FP64 = 0, XFMM_MIXED = 1 };
+enum { XFMM_FP64 = 0, XFMM_MIXED = 1, XFMM_FLOAT = 2 };
 enum {
   XFMM_OK = 0,
   XFMM_ERR_ARG = 1,    /* invalid argument / inconsistent tree */
@@ -88,7 +88,15 @@ typedef struct {
  * (P2M, P2L), the translation GEMMs (P2M/L2P conversions, M2M, L2L) and the
  * evaluation of local equivalent densities at particles (L2P, M2P).  The
  * environment variable XFMM_MIXED_STAGES (comma list of p2p, m2l, fft, far,
- * surf) overrides the FP32 stage set of XFMM_MIXED when a context is created. */
+ * surf) overrides the FP32 stage set of XFMM_MIXED when a context is created.
+ * XFMM_FLOAT is the counterpart of the reference's -DFLOAT build
+ * (include/exafmm_t.h:21-37): every evaluation stage computes in FP32 (P2P,
+ * M2L, P2M/P2L check potentials, L2P/M2P, the translation GEMMs on FP32
+ * MFMA); values stay FP64 in memory.  It expects operators whose
+ * pseudo-inverses drop singular values below 4e-8 max(S) (the reference's
+ * FLOAT EPS, exafmm_t.h:23), which the C++ / Python host layer precomputes
+ * for this mode: with untruncated pseudo-inverses FP32 arithmetic in P2M,
+ * L2P or the GEMMs loses the result. */
 int xfmm_create(int kernel, int p, double wavek_re, double wavek_im, int precision, int device,
                 xfmm_context** out);
 void xfmm_destroy(xfmm_context* ctx);

@@ -158,6 +158,34 @@ def test_non_adaptive_tree(require_gpu, kernel, p, dist, depth):
         assert rel_err(v[:, 1:], ref[:, 1:]) < (tol if prec == abi.FP64 else 1e-4)
 
 
+@pytest.mark.parametrize("kernel,p,dist,n", [("laplace", 10, "clusters", 30000), ("laplace", 8, "plummer", 20000),
+                                            ("modified_helmholtz", 6, "sphere", 20000), ("helmholtz", 5, "cube", 10000)])
+def test_float_mode(require_gpu, kernel, p, dist, n):
+    """FLOAT mode (the reference's -DFLOAT build: pseudo-inverses truncated at
+    4e-8 max(S), every evaluation stage including the translation GEMMs in
+    FP32): against the oracle evaluating the same operators in FP64 it carries
+    only FP32 rounding, and against the direct sum it stays near the FP64
+    path's accuracy."""
+    pr = make_problem(kernel, n, p, dist=dist, seed=12, precision=abi.FLOAT)
+    print("float case", kernel, p, dist, n, flush=True)
+    orc = O.Oracle(pr["kid"], p, pr["flat"], pr["ops"], pr["wavek"])
+    orc.set_charges(pr["qs"])
+    ref = orc.evaluate()
+    eng = _engine(pr, abi.FLOAT)
+    eng.upward_pass()
+    eng.downward_pass()
+    v = eng.values()
+    e0, e1 = rel_err(v[:, 0], ref[:, 0]), rel_err(v[:, 1:], ref[:, 1:])
+    tc = pr["flat"]["trg_coord"].reshape(-1, 3)
+    idx = np.arange(0, tc.shape[0], 13)
+    d = O.direct(pr["kid"], pr["flat"]["src_coord"].reshape(-1, 3), pr["qs"], tc[idx], pr["wavek"])
+    d0, d64 = rel_err(v[idx, 0], d[:, 0]), rel_err(ref[idx, 0], d[:, 0])
+    print("vs oracle phi %.2e grad %.2e; vs direct %.2e (oracle %.2e)" % (e0, e1, d0, d64))
+    # same bar as the mixed path (test_mixed_precision)
+    assert e0 < 1e-5 and e1 < 1e-4
+    assert d0 < max(3 * d64, 1e-5)
+
+
 @pytest.mark.parametrize("kernel,p,dist", [("laplace", 10, "clusters"), ("helmholtz", 5, "cube")])
 def test_mixed_hadamard_fp16_vs_fp32(require_gpu, monkeypatch, kernel, p, dist):
     """The default mixed Hadamard (split FP16 on the f16 matrix cores) and the
