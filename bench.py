@@ -3,10 +3,12 @@
 One step = upward_pass + downward_pass of the whole tree with charges already
 resident in HBM (the FMM evaluation phase, every operator including the
 near-field P2P, which each upward_pass starts anew on a second stream; the
-host tree/list/precompute setup is reported separately).  The headline `value` is the mixed-precision path
-(FP32 P2P and M2L transforms, M2L Hadamard in split FP16 with FP32 accumulation, FP64 pseudo-inverse
-solves and translations); the plain
-FP64 path is timed beside it.  Errors are relative L2 against an FP64 direct
+host tree/list/precompute setup is reported separately).  The headline `value` is the mixed-precision
+FLOAT mode: the operators of the reference's -DFLOAT build (pseudo-inverses truncated at 4e-8 max(S)),
+FP32 arithmetic in every evaluation stage (P2P, transforms, the M2L Hadamard in split FP16 with FP32
+accumulation, check potentials, L2P, the translation GEMMs on FP32 MFMA) with FP64 storage, coordinates and
+final sums.  Timed beside it: the mixed path with the full-precision operators (FP32 P2P and M2L, FP64
+pseudo-inverse solves and translations) and the plain FP64 path.  Errors are relative L2 against an FP64 direct
 sum on 2,000 seeded sample targets computed by the engine's direct-sum kernel.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--n 2000000] [--no-cpu-baseline]
@@ -38,6 +40,30 @@ PEAK = {"fp32_vector": 157.3e12, "fp64_vector": 78.6e12, "fp32_mfma": 157.3e12, 
         "fp16_mfma": 2.5e15, "hbm": 8.0e12}
 
 
+DTYPE = {"float": "f32 arithmetic in every evaluation stage (f16-split Hadamard, f32 MFMA translation GEMMs), "
+                  "f64 storage and setup; pseudo-inverses truncated at 4e-8 max(S)",
+         "mixed": "f32 P2P/M2L (f16-split Hadamard) + f64 pinv solves and translations",
+         "fp64": "f64"}
+PATHS = (("float", 2), ("mixed", 1), ("fp64", 0))  # abi.FLOAT, abi.MIXED, abi.FP64; the first is the headline
+
+
+def compare_fields(res, head):
+    """Extra fields of a JSON line: every path other than the headline (value, ms, errors) and the
+    headline / FP64 throughput ratio."""
+    out = {}
+    for name, r in res.items():
+        if name == head:
+            continue
+        out[name + "_value"] = r["value"]
+        out[name + "_ms_per_step"] = r["ms"]
+        out[name + "_err_phi"], out[name + "_err_grad"] = r["err"][0], r["err"][1]
+        if "kernels" in r:
+            out[name + "_kernels_ms"] = r["kernels"]
+    if "fp64" in res and head != "fp64":
+        out["mixed_vs_fp64"] = res[head]["value"] / res["fp64"]["value"]
+    return out
+
+
 def had16_active():
     """True when the mixed evaluation runs its Hadamard on split FP16 (the default stage set)."""
     st = os.environ.get("XFMM_MIXED_STAGES", "p2p,m2l,had16,fft")
@@ -63,7 +89,20 @@ def setup_problem(n, p, ncrit, seed):
     ops = fmm.operators()
     qs = fmm.sorted_charges(tree)
     t_flat = time.perf_counter() - t0
-    return dict(x=x, q=q, flat=flat, ops=ops, qs=qs, depth=fmm.depth, t_setup=t_setup, t_flat=t_flat)
+    return dict(x=x, q=q, flat=flat, ops=ops, ops_float=float_operators(fmm), qs=qs, depth=fmm.depth,
+                t_setup=t_setup, t_flat=t_flat)
+
+
+def float_operators(fmm):
+    """Operators of the FLOAT mode: the same precompute with the pseudo-inverses truncated at 4e-8 max(S)."""
+    from exafmm import abi
+    save = fmm.precision
+    fmm.precision = abi.FLOAT
+    fmm.precompute()
+    ops = fmm.operators()
+    fmm.precision = save
+    fmm.precompute()
+    return ops
 
 
 def p2p_pairs(flat):
@@ -77,7 +116,7 @@ def p2p_pairs(flat):
 def run_path(prob, p, precision, steps, warmup):
     from exafmm import abi
     eng = abi.Engine(abi.LAPLACE, p, 0.0, precision, 0)
-    eng.set_operators(prob["ops"])
+    eng.set_operators(prob["ops_float"] if precision == abi.FLOAT else prob["ops"])
     eng.set_tree(prob["flat"])
     eng.set_charges(prob["qs"])
     eng.synchronize()
@@ -197,14 +236,14 @@ def setup_generic(kid, n, p, ncrit, wavek, dist, charges, seed):
     flat = fmm.flat_tree(tree)
     ops = fmm.operators()
     qs = fmm.sorted_charges(tree)
-    return dict(kid=kid, p=p, wavek=wavek, flat=flat, ops=ops, qs=qs, depth=fmm.depth, n=n,
-                t_setup=time.perf_counter() - t0)
+    return dict(kid=kid, p=p, wavek=wavek, flat=flat, ops=ops, ops_float=float_operators(fmm), qs=qs,
+                depth=fmm.depth, n=n, t_setup=time.perf_counter() - t0)
 
 
 def engine_for(prob, precision):
     from exafmm import abi
     eng = abi.Engine(prob["kid"], prob["p"], prob["wavek"], precision, 0)
-    eng.set_operators(prob["ops"])
+    eng.set_operators(prob["ops_float"] if precision == abi.FLOAT else prob["ops"])
     eng.set_tree(prob["flat"])
     eng.set_charges(prob["qs"])
     return eng
@@ -231,7 +270,7 @@ def run_single_workload(a, name):
     prob = setup_generic(kid, n, p, ncrit, wavek, dist, charges, seed=0)
     log("setup %.1fs depth %d nodes %d" % (prob["t_setup"], prob["depth"], prob["flat"]["nnodes"]))
     res = {}
-    for pname, prec in (("mixed", abi.MIXED), ("fp64", abi.FP64)):
+    for pname, prec in PATHS:
         eng = engine_for(prob, prec)
         for _ in range(a.warmup):
             eng.upward_pass()
@@ -248,21 +287,21 @@ def run_single_workload(a, name):
         eng.close()
         res[pname] = dict(ms=1e3 * dt, value=n / dt, err=err, kernels={k: round(v, 4) for k, v in kt.items()})
         log("%s %s: %.3f ms/step err phi %.2e grad %.2e" % (name, pname, 1e3 * dt, err[0], err[1]))
-    m, f = res["mixed"], res["fp64"]
-    return {
+    m = res["float"]
+    line = {
         "metric": "particles/s, %s FMM evaluation" % name, "value": m["value"], "unit": "particles/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": m["ms"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 P2P/M2L (f16-split Hadamard) + f64 pinv solves" + (" (complex)" if kid == 1 else ""),
+        "dtype": DTYPE["float"] + (" (complex; complex GEMMs f64)" if kid == 1 else ""),
         "data": "synthetic (seeded %s, charges %s)" % (dist, charges),
         "config": {"workload": "%s_N%d_p%d_ncrit%d" % (name, n, p, ncrit), "kernel": ["laplace", "helmholtz",
                    "modified_helmholtz"][kid], "wavek": wavek, "N": n, "p": p, "ncrit": ncrit,
                    "depth": prob["depth"], "nodes": int(prob["flat"]["nnodes"])},
         "err_phi": m["err"][0], "err_grad": m["err"][1], "kernels_ms": m["kernels"],
-        "fp64_value": f["value"], "fp64_ms_per_step": f["ms"], "fp64_err_phi": f["err"][0],
-        "fp64_err_grad": f["err"][1], "fp64_kernels_ms": f["kernels"], "mixed_vs_fp64": m["value"] / f["value"],
         "setup_s": round(prob["t_setup"], 2), "roofline": None, "cpu_baseline": None,
     }
+    line.update(compare_fields(res, "float"))
+    return line
 
 
 def run_batched(a, nprob=64):
@@ -276,7 +315,7 @@ def run_batched(a, nprob=64):
     t_setup = time.perf_counter() - t0
     log("batched setup %.1fs, %d problems, %d particles" % (t_setup, nprob, int(ns.sum())))
     res = {}
-    for pname, prec in (("mixed", abi.MIXED), ("fp64", abi.FP64)):
+    for pname, prec in PATHS:
         engs = [engine_for(pr, prec) for pr in probs]
         for e in engs:
             e.graph_capture()
@@ -295,23 +334,24 @@ def run_batched(a, nprob=64):
         errs = [sample_error_generic(pr, e.values(), 200, seed=7) for pr, e in zip(probs, engs)]
         for e in engs:
             e.close()
-        res[pname] = dict(ms=1e3 * dt, value=float(ns.sum()) / dt, err_phi=max(x[0] for x in errs),
-                          err_grad=max(x[1] for x in errs))
-        log("batched %s: %.3f ms per batch, max err phi %.2e grad %.2e" % (pname, 1e3 * dt, res[pname]["err_phi"],
-                                                                           res[pname]["err_grad"]))
-    m, f = res["mixed"], res["fp64"]
-    return {
+        res[pname] = dict(ms=1e3 * dt, value=float(ns.sum()) / dt,
+                          err=(max(x[0] for x in errs), max(x[1] for x in errs)))
+        log("batched %s: %.3f ms per batch, max err phi %.2e grad %.2e" % (pname, 1e3 * dt, res[pname]["err"][0],
+                                                                           res[pname]["err"][1]))
+    m = res["float"]
+    line = {
         "metric": "particles/s, laplace_batched FMM evaluation (aggregate)", "value": m["value"],
         "unit": "particles/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": m["ms"],
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 P2P/M2L (f16-split Hadamard) + f64 pinv solves",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": DTYPE["float"],
         "data": "synthetic (seeded Plummer, scale radius 1, truncated at r=10; charges 1/N)",
         "config": {"workload": "laplace_batched_%dproblems_p8_ncrit64" % nprob, "kernel": "laplace",
                    "problems": nprob, "N_total": int(ns.sum()), "N_min": int(ns.min()), "N_max": int(ns.max()),
                    "p": 8, "ncrit": 64, "graph": "one captured evaluation per problem"},
-        "err_phi": m["err_phi"], "err_grad": m["err_grad"], "fp64_value": f["value"], "fp64_ms_per_step": f["ms"],
-        "fp64_err_phi": f["err_phi"], "fp64_err_grad": f["err_grad"], "mixed_vs_fp64": m["value"] / f["value"],
+        "err_phi": m["err"][0], "err_grad": m["err"][1],
         "setup_s": round(t_setup, 2), "roofline": None, "cpu_baseline": None,
     }
+    line.update(compare_fields(res, "float"))
+    return line
 
 
 def main():
@@ -326,7 +366,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
-    ap.add_argument("--mixed-only", action="store_true")
+    ap.add_argument("--mixed-only", action="store_true", help="time the headline mixed-precision (FLOAT) path only")
     ap.add_argument("--workload", default="laplace_uniform",
                     choices=["laplace_uniform", "laplace_batched"] + sorted(WORKLOADS))
     a = ap.parse_args()
@@ -357,9 +397,9 @@ def main():
     nfreq = n1 * n1 * (n1 // 2 + 1)
 
     res = {}
-    paths = [("fp64", abi.FP64)] if a.fp64_only else [("mixed", abi.MIXED), ("fp64", abi.FP64)]
+    paths = [("fp64", abi.FP64)] if a.fp64_only else list(PATHS)
     if a.mixed_only:
-        paths = [("mixed", abi.MIXED)]
+        paths = [PATHS[0]]
     for name, prec in paths:
         log("timing %s path: warmup %d steps %d" % (name, a.warmup, a.steps))
         r = run_path(prob, a.p, prec, a.steps, a.warmup)
@@ -368,9 +408,13 @@ def main():
             name, 1e3 * r["total"] / a.steps, r["err"][0], r["err"][1],
             {k: round(v, 3) for k, v in r["kernels"].items()}))
         del r["values"]
+        r["value"] = a.n / (r["total"] / a.steps)
+        r["ms"] = 1e3 * r["total"] / a.steps
+        r["kernels"] = {k: round(v, 4) for k, v in r["kernels"].items()}
         res[name] = r
 
-    head = res["mixed"] if "mixed" in res else res["fp64"]
+    hname = paths[0][0]
+    head = res[hname]
     ms_step = 1e3 * head["total"] / a.steps
     value = a.n / (head["total"] / a.steps)
     kt = head["kernels"]
@@ -379,10 +423,10 @@ def main():
     # concurrently on the second stream, so it is not one kernel's duration)
     cand = {"P2P": kt.get("P2P", 0.0), "HAD": kt.get("HAD", 0.0), "IFFT": kt.get("IFFT", 0.0)}
     dom = max(cand, key=cand.get)
-    mixed = "mixed" in res
+    mixed = hname != "fp64"
     if dom == "P2P":
         flops = P2P_FLOP_PER_PAIR * npairs
-        roof = {"kernel": "P2P near field (k_eval_targets, %s)" % ("FP32" if mixed else "FP64"),
+        roof = {"kernel": "P2P near field (k_p2p_smem, %s)" % ("FP32" if mixed else "FP64"),
                 "bound": "valu", "achieved": flops / (kt["P2P"] * 1e-3) / 1e12,
                 "peak": (PEAK["fp32_vector"] if mixed else PEAK["fp64_vector"]) / 1e12, "unit": "TFLOP/s",
                 "traffic": None, "algorithmic": "%d pairs x %d flop" % (npairs, P2P_FLOP_PER_PAIR)}
@@ -421,24 +465,20 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 P2P/M2L (f16-split Hadamard) + f64 pinv solves" if mixed else "f64",
+        "dtype": DTYPE[hname],
         "data": "synthetic (seeded uniform [0,1]^3, charges U[-1,1])",
         "config": {"workload": "laplace_fmm_N%d_p%d_ncrit%d_uniform" % (a.n, a.p, a.ncrit), "kernel": "laplace",
                    "N": a.n, "p": a.p, "ncrit": a.ncrit, "depth": prob["depth"], "nodes": int(flat["nnodes"]),
-                   "p2p_pairs": npairs, "m2l_pairs": m2l_pairs, "precision": "mixed" if mixed else "fp64"},
+                   "p2p_pairs": npairs, "m2l_pairs": m2l_pairs,
+                   "precision": {"float": "mixed: FLOAT mode (f32 evaluation, f64 storage)", "mixed": "mixed",
+                                 "fp64": "fp64"}[hname]},
         "err_phi": head["err"][0],
         "err_grad": head["err"][1],
-        "kernels_ms": {k: round(v, 4) for k, v in kt.items()},
+        "kernels_ms": kt,
         "setup_s": round(prob["t_setup"] + prob["t_flat"], 3),
         "roofline": roof,
     }
-    if mixed and "fp64" in res:
-        f = res["fp64"]
-        line["fp64_value"] = a.n / (f["total"] / a.steps)
-        line["fp64_ms_per_step"] = 1e3 * f["total"] / a.steps
-        line["fp64_err_phi"], line["fp64_err_grad"] = f["err"]
-        line["fp64_kernels_ms"] = {k: round(v, 4) for k, v in f["kernels"].items()}
-        line["mixed_vs_fp64"] = line["value"] / line["fp64_value"]
+    line.update(compare_fields(res, hname))
     if not a.no_cpu_baseline:
         log("cpu baseline (oracle) on N=%d" % a.cpu_sample)
         line["cpu_baseline"] = cpu_baseline(a.p, a.ncrit, a.cpu_sample)

@@ -277,6 +277,8 @@ void bind_fmm(py::module& m, py::class_<F>& c) {
       .def_readwrite("device", &F::device)
       .def("flat_tree", [](F& f, Tree<T>& t) { return flat_tree<T, F>(f, t); })
       .def("operators", [](F& f) { return operators<T, F>(f); })
+      .def("precompute", &F::precompute,
+           "recompute the operators for the current precision (FLOAT truncates the pseudo-inverses at 4e-8)")
       .def("sorted_charges", [](F& f, Tree<T>& t) { return sorted_charges<T, F>(f, t); })
       .def("sync_equiv_to_host", [](F& f, Tree<T>& t) { f.sync_equiv_to_host(t.nodes); })
       .def("release_device", &F::release_device);
