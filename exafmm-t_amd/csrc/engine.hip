@@ -99,6 +99,7 @@ struct xfmm_context {
   bool have_ops = false;
   int nlevels = 0, nm2l = 0;
   DevBuf<double> uc2e_uT, uc2e_vT, dc2e_uT, dc2e_vT, m2mcat, l2lcat;
+  DevBuf<float> uc2e_uT32, uc2e_vT32, dc2e_uT32, dc2e_vT32, m2mcat32, l2lcat32;  // f32_gemm (real kernels)
   DevBuf<double> m2l64;  // M2L operators in the expanded MFMA form, [L][nfreq][26][64][4]
   DevBuf<float> m2l32;   // FP32 form, or the split-FP16 form of the same size when f16_had
   DevBuf<float> m2l_hscale;  // f16_had: operator scale per [L][nfreq]
@@ -294,6 +295,13 @@ void check_ready(xfmm_context* c) {
   if (c->nm2l != 1 && c->nm2l < c->depth) throw ArgError{"xfmm: M2L operators cover fewer levels than the tree depth"};
 }
 
+// B operand of a translation GEMM: the FP32 copy for FP32 GEMMs of the real
+// kernels, else the FP64 operator; off = element offset.
+const void* gemm_op(const xfmm_context* c, const DevBuf<double>& d, const DevBuf<float>& f, size_t off) {
+  if (c->f32_gemm && !c->cplx) return f.ptr + off;
+  return d.ptr + off;
+}
+
 void op_P2M(xfmm_context* c) {
   // check potentials of every leaf, then up = alpha * UC2E_V (UC2E_U check)
   DevTree t = c->dev_tree();
@@ -305,10 +313,11 @@ void op_P2M(xfmm_context* c) {
     if (e <= b) continue;
     const int L = c->op_level(l);
     launch_gemm_rows(c->cplx, c->f32_gemm, e - b, c->nsurf, c->nsurf, c->check.ptr, c->leaf_slot_rows.ptr + b,
-                     c->uc2e_uT.ptr + L * ns2, c->tmp.ptr, c->leaf_slot_rows.ptr + b, 1.0, 0, c->stream);
-    launch_gemm_rows(c->cplx, c->f32_gemm, e - b, c->nsurf, c->nsurf, c->tmp.ptr, c->leaf_slot_rows.ptr + b,
-                     c->uc2e_vT.ptr + L * ns2, c->up.ptr, c->leaf_node_rows.ptr + b, c->leaf_scale(l), 0,
+                     gemm_op(c, c->uc2e_uT, c->uc2e_uT32, L * ns2), c->tmp.ptr, c->leaf_slot_rows.ptr + b, 1.0, 0,
                      c->stream);
+    launch_gemm_rows(c->cplx, c->f32_gemm, e - b, c->nsurf, c->nsurf, c->tmp.ptr, c->leaf_slot_rows.ptr + b,
+                     gemm_op(c, c->uc2e_vT, c->uc2e_vT32, L * ns2), c->up.ptr, c->leaf_node_rows.ptr + b,
+                     c->leaf_scale(l), 0, c->stream);
   }
 }
 
@@ -320,7 +329,7 @@ void op_M2M(xfmm_context* c) {
     const int L = c->op_level(l);
     // parent row = [child_0 .. child_7] (contiguous, K = 8 nsurf) x M2Mcat
     launch_gemm_rows(c->cplx, c->f32_gemm, e - b, c->nsurf, 8 * c->nsurf, c->up.ptr, c->nl_child_rows.ptr + b,
-                     c->m2mcat.ptr + L * blk, c->up.ptr, c->nl_rows.ptr + b, 1.0, 1, c->stream,
+                     gemm_op(c, c->m2mcat, c->m2mcat32, L * blk), c->up.ptr, c->nl_rows.ptr + b, 1.0, 1, c->stream,
                      c->gemm_scratch.ptr, c->gemm_scratch.n);
   }
 }
@@ -332,7 +341,7 @@ void op_L2L(xfmm_context* c) {
     if (e <= b) continue;
     const int L = c->op_level(l);
     launch_gemm_rows(c->cplx, c->f32_gemm, e - b, 8 * c->nsurf, c->nsurf, c->dn.ptr, c->nl_rows.ptr + b,
-                     c->l2lcat.ptr + L * blk,
+                     gemm_op(c, c->l2lcat, c->l2lcat32, L * blk),
                      c->dn.ptr, c->nl_child_rows.ptr + b, 1.0, 1, c->stream, c->gemm_scratch.ptr,
                      c->gemm_scratch.n);
   }
@@ -345,10 +354,11 @@ void op_L2P(xfmm_context* c) {
     if (e <= b) continue;
     const int L = c->op_level(l);
     launch_gemm_rows(c->cplx, c->f32_gemm, e - b, c->nsurf, c->nsurf, c->dn.ptr, c->leaf_node_rows.ptr + b,
-                     c->dc2e_uT.ptr + L * ns2, c->tmp.ptr, c->leaf_slot_rows.ptr + b, 1.0, 0, c->stream);
-    launch_gemm_rows(c->cplx, c->f32_gemm, e - b, c->nsurf, c->nsurf, c->tmp.ptr, c->leaf_slot_rows.ptr + b,
-                     c->dc2e_vT.ptr + L * ns2, c->dn.ptr, c->leaf_node_rows.ptr + b, c->leaf_scale(l), 0,
+                     gemm_op(c, c->dc2e_uT, c->dc2e_uT32, L * ns2), c->tmp.ptr, c->leaf_slot_rows.ptr + b, 1.0, 0,
                      c->stream);
+    launch_gemm_rows(c->cplx, c->f32_gemm, e - b, c->nsurf, c->nsurf, c->tmp.ptr, c->leaf_slot_rows.ptr + b,
+                     gemm_op(c, c->dc2e_vT, c->dc2e_vT32, L * ns2), c->dn.ptr, c->leaf_node_rows.ptr + b,
+                     c->leaf_scale(l), 0, c->stream);
   }
   DevTree t = c->dev_tree();
   launch_eval_targets(c->kind, c->f32_far, MODE_L2P, t, c->wavek(), c->w_l2p_eval.ptr, c->n_l2p_eval, c->t_l2p.ptr,
@@ -598,6 +608,18 @@ int xfmm_set_operators(xfmm_context* c, int nlevels, const double* uc2e_u, const
             }
     c->m2mcat.upload(mm, c->stream);
     c->l2lcat.upload(ll, c->stream);
+    // FP32 copies of the GEMM operators for the FP32 translation GEMMs
+    const std::pair<DevBuf<double>*, DevBuf<float>*> f32ops[] = {
+        {&c->uc2e_uT, &c->uc2e_uT32}, {&c->uc2e_vT, &c->uc2e_vT32}, {&c->dc2e_uT, &c->dc2e_uT32},
+        {&c->dc2e_vT, &c->dc2e_vT32}, {&c->m2mcat, &c->m2mcat32},   {&c->l2lcat, &c->l2lcat32}};
+    for (const auto& pr : f32ops) {
+      if (c->f32_gemm && !c->cplx) {
+        pr.second->alloc(pr.first->n);
+        launch_to_float(pr.first->ptr, pr.second->ptr, pr.first->n, c->stream);
+      } else {
+        pr.second->release();
+      }
+    }
     const size_t msz = size_t(nm2l) * NM2L * c->nfreq * 64 * 2;
     // reference layout uploaded, then expanded on the device into the per-lane
     // MFMA operand form (twice the size) used by the Hadamard kernel

@@ -89,20 +89,19 @@ __global__ __launch_bounds__(256) void k_gemm_rows_cplx(int m, int n, int k, con
   }
 }
 
-// FP64 MFMA path (v_mfma_f64_16x16x4_f64): BM x BN output tile per 256-thread
-// workgroup, the 4 waves as 2 x 2 sub-tiles of (BM/2) x (BN/2) (a grid of
-// 16 x 16 MFMA tiles each), K staged through LDS in steps of 16 with the next
+// MFMA path: BM x BN output tile per 256-thread workgroup, the 4 waves as
+// 2 x 2 sub-tiles of (BM/2) x (BN/2) (a grid of 16 x 16 MFMA tiles each), K
+// staged through LDS in steps of BK (16) with the next
 // step's global loads held in registers while the current step computes.
 // Tiles are ordered so that the column tiles of one BM-row band run on one
 // XCD (workgroup w goes to XCD w % 8), which then reads that band of A once
 // into its L2.  Optional split-K (grid.y): each split writes its partial
 // product to `partial` and k_gemm_reduce combines them in a fixed order.
-constexpr int GM_BK = 16;
-
 // Compute type of the MFMA path: FP64 (v_mfma_f64_16x16x4_f64, result
 // register q of lane group g = row g + 4q) or FP32 (v_mfma_f32_16x16x4_f32,
-// row 4g + q) for operators that tolerate it; operands are FP64 in memory
-// either way and converted while staged, results are stored as FP64.
+// row 4g + q) for operators that tolerate it; A (the vectors) is FP64 in
+// memory and rounded while staged, B (the operator) is read in the compute
+// type, results are stored as FP64.
 template <typename CT>
 struct GemmMfma;
 template <>
@@ -127,13 +126,13 @@ struct GemmMfma<float> {
 // epilogue.  BETA = false keeps the epilogue a plain store, so the
 // accumulators never leave the AGPRs before it (BETA = true costs occupancy
 // and is used only when no split-K scratch is available).
-template <int BM, int BN, bool BETA, typename CT>
+template <int BM, int BN, int BK, bool BETA, typename CT>
 __global__ __launch_bounds__(256) void k_gemm_rows_mfma64(int m, int n, int k, int kper, int ncol,
                                                          const double* __restrict__ A, const long* __restrict__ arow,
-                                                         const double* __restrict__ B, double* __restrict__ C,
+                                                         const CT* __restrict__ B, double* __restrict__ C,
                                                          const long* __restrict__ crow, double alpha,
                                                          double* __restrict__ partial) {
-  constexpr int BK = GM_BK, WM = BM / 2, WN = BN / 2, TA = WM / 16, TB = WN / 16;
+  constexpr int WM = BM / 2, WN = BN / 2, TA = WM / 16, TB = WN / 16;
   constexpr int NQA = BM * BK / 256, NQB = BK * BN / 256;  // loader elements per thread
   constexpr int ARS = 256 / BK, BKS = 256 / BN;            // loader row / k strides
   static_assert(BM % 32 == 0 && BN % 32 == 0 && BN <= 256 && NQA >= 1 && NQB >= 1, "tile shape");
@@ -176,14 +175,15 @@ __global__ __launch_bounds__(256) void k_gemm_rows_mfma64(int m, int n, int k, i
 #pragma unroll
   for (int q = 0; q < NQA; q++) arw[q] = arow_s[ar0 + ARS * q];
   const bool bcol_ok = col0 + bc < n;
-  double ra[NQA], rb[NQB];
+  double ra[NQA];
+  CT rb[NQB];
   auto fetch = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < NQA; q++) ra[q] = (arw[q] >= 0 && k0 + akk < kend) ? A[arw[q] + k0 + akk] : 0.0;
 #pragma unroll
     for (int q = 0; q < NQB; q++) {
       const int kb = bkk + BKS * q;
-      rb[q] = (bcol_ok && k0 + kb < kend) ? B[size_t(k0 + kb) * n + col0 + bc] : 0.0;
+      rb[q] = (bcol_ok && k0 + kb < kend) ? B[size_t(k0 + kb) * n + col0 + bc] : CT(0);
     }
   };
   if (kbeg < kend) fetch(kbeg);
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(256) void k_gemm_rows_mfma64(int m, int n, int k, i
 #pragma unroll
     for (int q = 0; q < NQA; q++) As[akk][ar0 + ARS * q] = CT(scale * ra[q]);
 #pragma unroll
-    for (int q = 0; q < NQB; q++) Bs[bkk + BKS * q][bc] = CT(rb[q]);
+    for (int q = 0; q < NQB; q++) Bs[bkk + BKS * q][bc] = rb[q];
     __syncthreads();
     if (k0 + BK < kend) fetch(k0 + BK);
 #pragma unroll
@@ -233,6 +233,17 @@ __global__ __launch_bounds__(256) void k_gemm_rows_mfma64(int m, int n, int k, i
     }
 }
 
+__global__ void k_to_float(const double* __restrict__ in, float* __restrict__ out, size_t n) {
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = float(in[i]);
+}
+
+void launch_to_float(const double* in, float* out, size_t n, hipStream_t s) {
+  if (n == 0) return;
+  const unsigned blocks = unsigned(std::min<size_t>((n + 255) / 256, 65536));
+  hipLaunchKernelGGL(k_to_float, dim3(blocks), dim3(256), 0, s, in, out, n);
+}
+
 __global__ void k_gemm_reduce(int m, int n, int nsplit, const double* __restrict__ partial, double* __restrict__ C,
                               const long* __restrict__ crow, double alpha, int beta) {
   const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -270,21 +281,27 @@ size_t gemm_scratch_doubles(int m, int n, int k) {
   return split > 1 ? size_t(split) * m * n : 0;
 }
 
+// K step (measured on 32768x488x488 and 4096x488x3904: FP32 with 32 needs
+// more registers and drops to 2 waves per SIMD, 5-7 % slower than 16)
+template <typename CT>
+constexpr int gemm_bk() { return 16; }
+
 template <int BM, int BN, bool BETA, typename CT>
-static void launch_mfma(int m, int n, int k, int split, const double* A, const long* arow, const double* B, double* C,
+static void launch_mfma(int m, int n, int k, int split, const double* A, const long* arow, const CT* B, double* C,
                         const long* crow, double alpha, double* partial, hipStream_t s) {
+  constexpr int BK = gemm_bk<CT>();
   int kper = (k + split - 1) / split;
-  kper = (kper + GM_BK - 1) / GM_BK * GM_BK;
+  kper = (kper + BK - 1) / BK * BK;
   const int ncol = (n + BN - 1) / BN;
   const int nband = (m + BM - 1) / BM;
   const int nb = nband >= 8 ? (nband + 7) / 8 * 8 : nband;
   dim3 grid(unsigned(nb * ncol), split);
-  hipLaunchKernelGGL((k_gemm_rows_mfma64<BM, BN, BETA, CT>), grid, dim3(256), 0, s, m, n, k, kper, ncol, A, arow,
-                     B, C, crow, alpha, partial);
+  hipLaunchKernelGGL((k_gemm_rows_mfma64<BM, BN, BK, BETA, CT>), grid, dim3(256), 0, s, m, n, k, kper, ncol, A,
+                     arow, B, C, crow, alpha, partial);
 }
 
 template <int BM, int BN, typename CT>
-static void launch_tile(int m, int n, int k, int split, bool part, const double* A, const long* arow, const double* B,
+static void launch_tile(int m, int n, int k, int split, bool part, const double* A, const long* arow, const CT* B,
                         double* C, const long* crow, double alpha, int beta, double* scratch, hipStream_t s) {
   // (the 128 x 128 += epilogue is never built: it would cost occupancy;
   // launch_real picks 128 x 128 only for split-K or plain stores)
@@ -300,7 +317,7 @@ static void launch_tile(int m, int n, int k, int split, bool part, const double*
 }
 
 template <typename CT>
-static void launch_real(int m, int n, int k, const double* A, const long* arow, const double* B, double* C,
+static void launch_real(int m, int n, int k, const double* A, const long* arow, const CT* B, double* C,
                         const long* crow, double alpha, int beta, hipStream_t s, double* scratch,
                         size_t scratch_doubles) {
   // an accumulating product without split-K uses the 64 x 64 tile, whose
@@ -319,12 +336,14 @@ static void launch_real(int m, int n, int k, const double* A, const long* arow, 
   }
 }
 
-void launch_gemm_rows(bool cplx, bool f32, int m, int n, int k, const double* A, const long* arow, const double* B,
+void launch_gemm_rows(bool cplx, bool f32, int m, int n, int k, const double* A, const long* arow, const void* Bv,
                       double* C, const long* crow, double alpha, int beta, hipStream_t s, double* scratch,
                       size_t scratch_doubles) {
+  const double* B = static_cast<const double*>(Bv);
   if (m <= 0 || n <= 0) return;
   if (!cplx) {
-    if (f32) launch_real<float>(m, n, k, A, arow, B, C, crow, alpha, beta, s, scratch, scratch_doubles);
+    if (f32) launch_real<float>(m, n, k, A, arow, static_cast<const float*>(Bv), C, crow, alpha, beta, s, scratch,
+                                scratch_doubles);
     else launch_real<double>(m, n, k, A, arow, B, C, crow, alpha, beta, s, scratch, scratch_doubles);
   } else {
     constexpr int BM = 64, BN = 64;

@@ -68,10 +68,13 @@ void launch_direct(int kind, bool mixed, bool cplx, Wavek k, const double* src, 
 // FP32 while staged, FP32 accumulation, FP64 results); when the output has few
 // tiles the K range is split over workgroups, partial products go to
 // `scratch` (at least gemm_scratch_doubles(m, n, k) doubles) and are reduced
-// in a fixed order.  Complex GEMMs are FP64 (f32 ignored).
-void launch_gemm_rows(bool cplx, bool f32, int m, int n, int k, const double* A, const long* arow, const double* B,
+// in a fixed order.  B is float (f32 and real) or double; complex GEMMs are
+// FP64 (f32 ignored).
+void launch_gemm_rows(bool cplx, bool f32, int m, int n, int k, const double* A, const long* arow, const void* B,
                       double* C, const long* crow, double alpha, int beta, hipStream_t s,
                       double* scratch = nullptr, size_t scratch_doubles = 0);
+// out[i] = float(in[i]) for i < n
+void launch_to_float(const double* in, float* out, size_t n, hipStream_t s);
 size_t gemm_scratch_doubles(int m, int n, int k);
 
 // M2L stages.
