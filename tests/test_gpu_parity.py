@@ -3,7 +3,8 @@
 Bar (FP64 path): relative L2 difference <= 1e-10 against the oracle on the same
 tree, operators and charges, after every operator of the evaluation phase and
 for the full upward + downward pass.  Mixed path: within 1e-5 of the FP64
-oracle and of the FP64 direct sum at the sizes used here.
+oracle and of the FP64 direct sum at the sizes used here.  FLOAT mode: within
+1e-5 of the oracle evaluating the same (truncated) operators in FP64.
 """
 import numpy as np
 import pytest
@@ -264,11 +265,11 @@ def test_linearity_and_repeatability(require_gpu):
 
 
 @pytest.mark.parametrize("kernel,precision", [("laplace", abi.MIXED), ("modified_helmholtz", abi.FP64),
-                                              ("helmholtz", abi.MIXED)])
+                                              ("helmholtz", abi.MIXED), ("laplace", abi.FLOAT)])
 def test_graph_replay_matches_eager(require_gpu, kernel, precision):
     """A captured evaluation replays to the eager values, replay after replay, with eager evaluations and new
     charges in between (the graph holds kernels only: its clears are fill kernels)."""
-    pr = make_problem(kernel, 6000, 6, ncrit=32, dist="plummer", seed=21)
+    pr = make_problem(kernel, 6000, 6, ncrit=32, dist="plummer", seed=21, precision=precision)
     print("graph case", kernel, precision, flush=True)
     eng = _engine(pr, precision)
     rng = np.random.default_rng(8)
