@@ -133,7 +133,8 @@ struct xfmm_context {
   DevBuf<char> fft_in, fft_out;
   // buffers
   DevBuf<double> up, dn, trg_value, check, tmp, gemm_scratch, near;
-  DevBuf<double> src_rel, src_pack, surf_tab;  // P2P sources relative to their leaf centre; packed (x, y, z, q)
+  DevBuf<double> src_rel, src_pack, surf_tab;
+  DevBuf<float> surf_tab32;  // P2P sources relative to their leaf centre; packed (x, y, z, q)
 
   // ---- timing
   hipEvent_t ev[24] = {};
@@ -171,6 +172,7 @@ struct xfmm_context {
     t.nsrcs = nsrcs;
     t.src_rel = src_rel.ptr;
     t.surf_tab = surf_tab.ptr;
+    t.surf_tab32 = surf_tab32.ptr;
     t.src_pack = src_pack.ptr;
     t.up = up.ptr;
     t.dn = dn.ptr;
@@ -744,6 +746,8 @@ int xfmm_set_tree(xfmm_context* c, const xfmm_tree* t) {
           for (int d = 0; d < 3; d++) tab[(size_t(l) * c->nsurf + i) * 4 + d] = us[3 * size_t(i) + d] * b;
       }
       c->surf_tab.upload(tab, st);
+      c->surf_tab32.upload(std::vector<float>(tab.begin(), tab.end()), st);
+      HIPCHK(hipStreamSynchronize(st));  // the FP32 copy is a temporary
     }
     c->trg.upload(t->trg_coord, size_t(t->ntrgs) * 3, st);
     c->p2p_off.upload(t->p2p_offset, nn + 1, st);

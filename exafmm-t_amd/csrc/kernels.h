@@ -31,6 +31,7 @@ struct DevTree {
   const double* unit_surf = nullptr;  // [nsurf][3] unit-cube surface (reference point order)
   const double* src_rel = nullptr;    // [nsrcs][3] source positions relative to their leaf centre
   const double* surf_tab = nullptr;   // [depth + 1][nsurf][4] L2P equivalent surface (radius 2.95 r_l), (x, y, z, 0) relative to the centre
+  const float* surf_tab32 = nullptr;  // the same in FP32 (FP32 L2P)
   void* src_pack = nullptr;           // [nsrcs] (x, y, z, q) relative to the leaf centre, in the P2P type
   double* up = nullptr;               // T[nnodes][nsurf]
   double* dn = nullptr;               // T[nnodes][nsurf]

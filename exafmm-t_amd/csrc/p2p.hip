@@ -304,13 +304,18 @@ template <int KIND, typename R>
 __global__ __launch_bounds__(64) void k_l2p_smem(DevTree t, Wavek wk, const int* __restrict__ work) {
   constexpr int NA = 4;
   typedef const double __attribute__((address_space(4))) * CD;
+  // surface points in the arithmetic type (no per-source conversions)
+  typedef const R __attribute__((address_space(4))) * CRT;
   const int leaf = work[2 * blockIdx.x], t0 = TILE * work[2 * blockIdx.x + 1];
   const int lane = threadIdx.x;
   const double cx = t.center[3 * leaf], cy = t.center[3 * leaf + 1], cz = t.center[3 * leaf + 2];
   const int tb = t.trg_begin[leaf], nt = min(TILE, t.trg_count[leaf] - t0), ns = t.nsurf;
   const R kr = R(wk.re), ki = R(wk.im);
   const size_t tab0 = size_t(t.level[leaf]) * ns;
-  const CD sp = (CD)(t.surf_tab) + tab0 * 4;
+  const R* tabr;
+  if constexpr (sizeof(R) == 4) tabr = reinterpret_cast<const R*>(t.surf_tab32);
+  else tabr = reinterpret_cast<const R*>(t.surf_tab);
+  const CRT sp = (CRT)(tabr) + tab0 * 4;
   const CD eq = (CD)(t.dn) + size_t(leaf) * ns;
   const bool split = 2 * nt <= TILE;
   const int k = split ? TILE / nt : 1;
@@ -350,14 +355,13 @@ __global__ __launch_bounds__(64) void k_l2p_smem(DevTree t, Wavek wk, const int*
     for (; j + 4 <= m; j += 4) {
 #pragma unroll
       for (int u = 0; u < 4; u++) {
-        const CD v = sp + 4 * (s0 + j + u);
-        pair_grad<KIND, R, RSQ_SEP>(R(v[0]) - tx, R(v[1]) - ty, R(v[2]) - tz, R(eq[s0 + j + u]), R(0), kr, ki,
-                                    part);
+        const CRT v = sp + 4 * (s0 + j + u);
+        pair_grad<KIND, R, RSQ_SEP>(v[0] - tx, v[1] - ty, v[2] - tz, R(eq[s0 + j + u]), R(0), kr, ki, part);
       }
     }
     for (; j < m; j++) {
-      const CD v = sp + 4 * (s0 + j);
-      pair_grad<KIND, R, RSQ_SEP>(R(v[0]) - tx, R(v[1]) - ty, R(v[2]) - tz, R(eq[s0 + j]), R(0), kr, ki, part);
+      const CRT v = sp + 4 * (s0 + j);
+      pair_grad<KIND, R, RSQ_SEP>(v[0] - tx, v[1] - ty, v[2] - tz, R(eq[s0 + j]), R(0), kr, ki, part);
     }
     fold_tile<R, NA>(acc, comp, part);
   }
