@@ -99,6 +99,9 @@ struct xfmm_context {
   bool have_ops = false;
   int nlevels = 0, nm2l = 0;
   DevBuf<double> uc2e_uT, uc2e_vT, dc2e_uT, dc2e_vT, m2mcat, l2lcat;
+  // numerical rank of the upward / downward pseudo-inverse per operator level
+  // (uc2e_uT / dc2e_uT hold only the first r_up / r_dn columns, row stride r)
+  std::vector<int> r_up, r_dn;
   DevBuf<float> uc2e_uT32, uc2e_vT32, dc2e_uT32, dc2e_vT32, m2mcat32, l2lcat32;  // f32_gemm (real kernels)
   DevBuf<double> m2l64;  // M2L operators in the expanded MFMA form, [L][nfreq][26][64][4]
   DevBuf<float> m2l32;   // FP32 form, or the split-FP16 form of the same size when f16_had
@@ -314,10 +317,13 @@ void op_P2M(xfmm_context* c) {
     const int b = c->leaf_level_off[l], e = c->leaf_level_off[l + 1];
     if (e <= b) continue;
     const int L = c->op_level(l);
-    launch_gemm_rows(c->cplx, c->f32_gemm, e - b, c->nsurf, c->nsurf, c->check.ptr, c->leaf_slot_rows.ptr + b,
+    // only the first r singular directions carry a nonzero S^+ entry: the
+    // first product computes those r coefficients, the second sums over them
+    const int r = c->r_up[L];
+    launch_gemm_rows(c->cplx, c->f32_gemm, e - b, r, c->nsurf, c->check.ptr, c->leaf_slot_rows.ptr + b,
                      gemm_op(c, c->uc2e_uT, c->uc2e_uT32, L * ns2), c->tmp.ptr, c->leaf_slot_rows.ptr + b, 1.0, 0,
                      c->stream);
-    launch_gemm_rows(c->cplx, c->f32_gemm, e - b, c->nsurf, c->nsurf, c->tmp.ptr, c->leaf_slot_rows.ptr + b,
+    launch_gemm_rows(c->cplx, c->f32_gemm, e - b, c->nsurf, r, c->tmp.ptr, c->leaf_slot_rows.ptr + b,
                      gemm_op(c, c->uc2e_vT, c->uc2e_vT32, L * ns2), c->up.ptr, c->leaf_node_rows.ptr + b,
                      c->leaf_scale(l), 0, c->stream);
   }
@@ -355,10 +361,11 @@ void op_L2P(xfmm_context* c) {
     const int b = c->leaf_level_off[l], e = c->leaf_level_off[l + 1];
     if (e <= b) continue;
     const int L = c->op_level(l);
-    launch_gemm_rows(c->cplx, c->f32_gemm, e - b, c->nsurf, c->nsurf, c->dn.ptr, c->leaf_node_rows.ptr + b,
+    const int r = c->r_dn[L];
+    launch_gemm_rows(c->cplx, c->f32_gemm, e - b, r, c->nsurf, c->dn.ptr, c->leaf_node_rows.ptr + b,
                      gemm_op(c, c->dc2e_uT, c->dc2e_uT32, L * ns2), c->tmp.ptr, c->leaf_slot_rows.ptr + b, 1.0, 0,
                      c->stream);
-    launch_gemm_rows(c->cplx, c->f32_gemm, e - b, c->nsurf, c->nsurf, c->tmp.ptr, c->leaf_slot_rows.ptr + b,
+    launch_gemm_rows(c->cplx, c->f32_gemm, e - b, c->nsurf, r, c->tmp.ptr, c->leaf_slot_rows.ptr + b,
                      gemm_op(c, c->dc2e_vT, c->dc2e_vT32, L * ns2), c->dn.ptr, c->leaf_node_rows.ptr + b,
                      c->leaf_scale(l), 0, c->stream);
   }
@@ -587,12 +594,42 @@ int xfmm_set_operators(xfmm_context* c, int nlevels, const double* uc2e_u, const
             for (int w = 0; w < vw; w++)
               dst[(L * ns2 + size_t(j) * ns + i) * vw + w] = src[(L * ns2 + size_t(i) * ns + j) * vw + w];
     };
+    // Rank of C2E_V = V S^+ per level: 1 + its last column that is not
+    // exactly zero (S^+ holds 0 for the truncated singular values, which
+    // come last).  The products with the columns past it are exact zeros, so
+    // the pseudo-inverse is applied as (first r rows of C2E_U) then (first r
+    // columns of C2E_V) with the same result.
+    auto rank_of = [&](const double* v, std::vector<int>& r) {
+      r.assign(nlevels, 1);
+      for (int L = 0; L < nlevels; L++)
+        for (int i = 0; i < ns; i++)
+          for (int j = ns - 1; j >= r[L]; j--) {
+            bool nz = false;
+            for (int w = 0; w < vw; w++) nz |= v[(L * ns2 + size_t(i) * ns + j) * vw + w] != 0.0;
+            if (nz) {
+              r[L] = j + 1;
+              break;
+            }
+          }
+    };
+    rank_of(uc2e_v, c->r_up);
+    rank_of(dc2e_v, c->r_dn);
+    // keep the first r columns of each level's transposed C2E_U, row stride r
+    auto compact = [&](std::vector<double>& d, const std::vector<int>& r) {
+      for (int L = 0; L < nlevels; L++)
+        for (int k = 0; k < ns; k++)
+          for (int n = 0; n < r[L]; n++)
+            for (int w = 0; w < vw; w++)
+              d[(L * ns2 + size_t(k) * r[L] + n) * vw + w] = d[(L * ns2 + size_t(k) * ns + n) * vw + w];
+    };
     std::vector<double> h;
     transpose_set(uc2e_u, h);
+    compact(h, c->r_up);
     c->uc2e_uT.upload(h, c->stream);
     transpose_set(uc2e_v, h);
     c->uc2e_vT.upload(h, c->stream);
     transpose_set(dc2e_u, h);
+    compact(h, c->r_dn);
     c->dc2e_uT.upload(h, c->stream);
     transpose_set(dc2e_v, h);
     c->dc2e_vT.upload(h, c->stream);

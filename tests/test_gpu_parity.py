@@ -187,6 +187,36 @@ def test_float_mode(require_gpu, kernel, p, dist, n):
     assert d0 < max(3 * d64, 1e-5)
 
 
+@pytest.mark.parametrize("kernel,p", [("laplace", 6), ("modified_helmholtz", 5), ("helmholtz", 5)])
+def test_pseudo_inverse_rank(require_gpu, kernel, p):
+    """P2M and L2P apply each pseudo-inverse over its first r singular
+    directions only, r = 1 + the last column of C2E_V (per operator level)
+    that is not exactly zero.  Operators whose C2E_V columns past odd,
+    level-dependent ranks are zeroed: the FP64 path equals the oracle on the
+    same operators to 1e-10 and the FLOAT path stays within the FP32 bar."""
+    pr = make_problem(kernel, 5000, p, dist="cube", seed=21)
+    ns = 6 * (p - 1) ** 2 + 2
+    ops = dict(pr["ops"])
+    for key, r0 in (("uc2e_v", 37), ("dc2e_v", 53)):
+        m = np.array(ops[key]).reshape(ops["nlevels"], ns, ns)
+        for L in range(m.shape[0]):
+            m[L, :, r0 + 6 * L:] = 0
+        ops[key] = m.reshape(-1)
+    pr = dict(pr, ops=ops)
+    orc = O.Oracle(pr["kid"], p, pr["flat"], ops, pr["wavek"])
+    orc.set_charges(pr["qs"])
+    ref = orc.evaluate()
+    for prec, tol in ((abi.FP64, FP64_TOL), (abi.FLOAT, 1e-5)):
+        eng = _engine(pr, prec)
+        eng.upward_pass()
+        eng.downward_pass()
+        v = eng.values()
+        eng.close()
+        e0, e1 = rel_err(v[:, 0], ref[:, 0]), rel_err(v[:, 1:], ref[:, 1:])
+        print(kernel, "precision", prec, "phi %.2e grad %.2e" % (e0, e1), flush=True)
+        assert e0 < tol and e1 < (tol if prec == abi.FP64 else 1e-4)
+
+
 @pytest.mark.parametrize("kernel,p,dist", [("laplace", 10, "clusters"), ("helmholtz", 5, "cube")])
 def test_mixed_hadamard_fp16_vs_fp32(require_gpu, monkeypatch, kernel, p, dist):
     """The default mixed Hadamard (split FP16 on the f16 matrix cores) and the
