@@ -354,6 +354,53 @@ def run_batched(a, nprob=64):
     return line
 
 
+def roofline(dom, ms, mixed, h16, npairs, m2l_pairs, nfreq, nsrc, ntrg):
+    """Roofline of the dominant kernel `dom` ("P2P", "HAD", "FFT" or "IFFT") of
+    one evaluation that took `ms` milliseconds.  For the Hadamard the bound is
+    min(MFMA peak, AI x HBM bandwidth): whichever of the two times
+    (algorithmic flops / peak, algorithmic bytes / HBM peak) is longer names
+    it, and the other view is reported beside it.  `traffic` is left to the
+    caller (measured counters)."""
+    t = ms * 1e-3
+    rb = 4 if mixed else 8
+    if dom == "P2P":
+        flops = P2P_FLOP_PER_PAIR * npairs
+        roof = {"kernel": "P2P near field (k_p2p_smem, %s)" % ("FP32" if mixed else "FP64"),
+                "bound": "valu", "achieved": flops / t / 1e12,
+                "peak": (PEAK["fp32_vector"] if mixed else PEAK["fp64_vector"]) / 1e12, "unit": "TFLOP/s",
+                "traffic": None, "algorithmic": "%d pairs x %d flop" % (npairs, P2P_FLOP_PER_PAIR)}
+    elif dom == "HAD":
+        flops = HAD_FLOP_PER_PAIR_FREQ * m2l_pairs * nfreq
+        name = "k_hadamard_h, split FP16 on the f16 MFMA (3 f16 products per FP32-equivalent product)" \
+            if h16 else "k_hadamard, %s" % ("FP32" if mixed else "FP64")
+        peak = PEAK["fp16_mfma"] / 3 if h16 else (PEAK["fp32_mfma"] if mixed else PEAK["fp64_mfma"])
+        # Laplace is scale-invariant: one set of 26 operators serves every level
+        byts = (nsrc + ntrg) * nfreq * 8 * 2 * rb + 26 * nfreq * 128 * rb
+        mfma_view = {"achieved": flops / t / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+                     "frac": flops / t / peak,
+                     "peak_note": "f16 MFMA 2500 TFLOP/s / 3 products" if h16 else "dense MFMA peak"}
+        hbm_view = {"achieved": byts / t / 1e9, "peak": PEAK["hbm"] / 1e9, "unit": "GB/s",
+                    "frac": byts / t / PEAK["hbm"]}
+        alg = "%d pairs x %d freq x %d flop; %d source + %d target nodes x %d freq x 8 children complex " \
+              "(%d B reals) + 26 operators x %d freq x 128 reals" % (
+                  m2l_pairs, nfreq, HAD_FLOP_PER_PAIR_FREQ, nsrc, ntrg, nfreq, rb, nfreq)
+        if byts / PEAK["hbm"] >= flops / peak:
+            roof = dict(hbm_view, bound="hbm", kernel="M2L hadamard_product (%s)" % name, traffic=None,
+                        algorithmic=alg, algorithmic_bytes=byts, mfma=mfma_view)
+        else:
+            roof = dict(mfma_view, bound="mfma", kernel="M2L hadamard_product (%s)" % name, traffic=None,
+                        algorithmic=alg, algorithmic_flops=flops, hbm=hbm_view)
+    else:
+        nslots = nsrc if dom == "FFT" else ntrg
+        byts = nslots * nfreq * 8 * 2 * rb
+        roof = {"kernel": "M2L %s" % dom, "bound": "hbm", "achieved": byts / t / 1e9,
+                "peak": PEAK["hbm"] / 1e9, "unit": "GB/s", "traffic": None,
+                "algorithmic": "%d nodes x %d freq x 8 children complex" % (nslots, nfreq)}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["ms"] = ms
+    return roof
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -424,52 +471,9 @@ def main():
     cand = {"P2P": kt.get("P2P", 0.0), "HAD": kt.get("HAD", 0.0), "IFFT": kt.get("IFFT", 0.0)}
     dom = max(cand, key=cand.get)
     mixed = hname != "fp64"
-    if dom == "P2P":
-        flops = P2P_FLOP_PER_PAIR * npairs
-        roof = {"kernel": "P2P near field (k_p2p_smem, %s)" % ("FP32" if mixed else "FP64"),
-                "bound": "valu", "achieved": flops / (kt["P2P"] * 1e-3) / 1e12,
-                "peak": (PEAK["fp32_vector"] if mixed else PEAK["fp64_vector"]) / 1e12, "unit": "TFLOP/s",
-                "traffic": None, "algorithmic": "%d pairs x %d flop" % (npairs, P2P_FLOP_PER_PAIR)}
-    elif dom == "HAD":
-        # roofline of the Hadamard = min(MFMA peak, AI x HBM bandwidth): whichever
-        # of the two times (algorithmic flops / peak, algorithmic bytes / HBM peak)
-        # is longer names the bound; the other view is reported beside it
-        flops = HAD_FLOP_PER_PAIR_FREQ * m2l_pairs * nfreq
-        h16 = mixed and had16_active()
-        name = "k_hadamard_h, split FP16 on the f16 MFMA (3 f16 products per FP32-equivalent product)" \
-            if h16 else "k_hadamard, %s" % ("FP32" if mixed else "FP64")
-        peak = PEAK["fp16_mfma"] / 3 if h16 else (PEAK["fp32_mfma"] if mixed else PEAK["fp64_mfma"])
-        rb = 4 if mixed else 8
-        nsrc = int(np.unique(flat["m2l_source"][flat["m2l_source"] >= 0]).size)
-        ntrg = int(np.sum(np.any(flat["m2l_source"].reshape(-1, 26) >= 0, axis=1)))
-        # Laplace is scale-invariant: one set of 26 operators serves every level
-        byts = (nsrc + ntrg) * nfreq * 8 * 2 * rb + 26 * nfreq * 128 * rb
-        t = kt["HAD"] * 1e-3
-        mfma_view = {"achieved": flops / t / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
-                     "frac": flops / t / peak,
-                     "peak_note": "f16 MFMA 2500 TFLOP/s / 3 products" if h16 else "dense MFMA peak"}
-        hbm_view = {"achieved": byts / t / 1e9, "peak": PEAK["hbm"] / 1e9, "unit": "GB/s",
-                    "frac": byts / t / PEAK["hbm"]}
-        alg = "%d pairs x %d freq x %d flop; %d source + %d target nodes x %d freq x 8 children complex " \
-              "(%d B reals) + 26 operators x %d freq x 128 reals" % (
-                  m2l_pairs, nfreq, HAD_FLOP_PER_PAIR_FREQ, nsrc, ntrg, nfreq, rb, nfreq)
-        if byts / PEAK["hbm"] >= flops / peak:
-            roof = dict(hbm_view, bound="hbm", kernel="M2L hadamard_product (%s)" % name, traffic=None,
-                        algorithmic=alg, algorithmic_bytes=byts, mfma=mfma_view)
-        else:
-            roof = dict(mfma_view, bound="mfma", kernel="M2L hadamard_product (%s)" % name, traffic=None,
-                        algorithmic=alg, algorithmic_flops=flops, hbm=hbm_view)
-    else:
-        rb = 4 if mixed else 8
-        nsrc = int(np.unique(flat["m2l_source"][flat["m2l_source"] >= 0]).size)
-        ntrg = int(np.sum(np.any(flat["m2l_source"].reshape(-1, 26) >= 0, axis=1)))
-        nslots = nsrc if dom == "FFT" else ntrg
-        byts = nslots * nfreq * 8 * 2 * rb
-        roof = {"kernel": "M2L %s" % dom, "bound": "hbm", "achieved": byts / (kt[dom] * 1e-3) / 1e9,
-                "peak": PEAK["hbm"] / 1e9, "unit": "GB/s", "traffic": None,
-                "algorithmic": "%d nodes x %d freq x 8 children complex" % (nslots, nfreq)}
-    roof["frac"] = roof["achieved"] / roof["peak"]
-    roof["ms"] = cand[dom]
+    nsrc = int(np.unique(flat["m2l_source"][flat["m2l_source"] >= 0]).size)
+    ntrg = int(np.sum(np.any(flat["m2l_source"].reshape(-1, 26) >= 0, axis=1)))
+    roof = roofline(dom, cand[dom], mixed, mixed and had16_active(), npairs, m2l_pairs, nfreq, nsrc, ntrg)
     roof["traffic"] = measured_traffic(dom, mixed, a)
 
     line = {
