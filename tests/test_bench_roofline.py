@@ -1,8 +1,9 @@
-"""bench.py's roofline of the dominant kernel (CPU only): the Hadamard's bound
-is the longer of its two roofline times, with the figures of the N=2M, p=10
-Laplace tree (104,368 M2L pairs, 4,680 source and 4,680 target nodes, 4,400
-frequencies) worked out by hand."""
+"""bench.py on the CPU: the roofline of the dominant kernel (the Hadamard's
+bound is the longer of its two roofline times, with the figures of the N=2M,
+p=10 Laplace tree — 104,368 M2L pairs, 4,680 source and 4,680 target nodes,
+4,400 frequencies — worked out by hand) and the refusal of multi-GPU runs."""
 import os
+import subprocess
 import sys
 
 import pytest
@@ -47,3 +48,12 @@ def test_transform_rows_are_hbm_bound():
     r = bench.roofline("IFFT", 0.6, True, True, 0, PAIRS, NFREQ, NSRC, 4000)
     assert r["bound"] == "hbm"
     assert r["achieved"] == pytest.approx(4000 * 4400 * 64 / 0.6e-3 / 1e9)
+
+
+@pytest.mark.parametrize("argv,env", [(["--gpus", "2"], {}), (["--gpus", "1"], {"WORLD_SIZE": "8"})])
+def test_more_than_one_gpu_is_refused(argv, env):
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py")] + argv, capture_output=True, text=True,
+                       env=dict(os.environ, **env), timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
