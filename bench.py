@@ -4,12 +4,13 @@ One step = upward_pass + downward_pass of the whole tree with charges already
 resident in HBM (the FMM evaluation phase, every operator including the
 near-field P2P, which each upward_pass starts anew on a second stream; the
 host tree/list/precompute setup is reported separately).  The headline `value` is the mixed-precision
-FLOAT mode: the operators of the reference's -DFLOAT build (pseudo-inverses truncated at 4e-8 max(S)),
-FP32 arithmetic in every evaluation stage (P2P, transforms, the M2L Hadamard in split FP16 with FP32
-accumulation, check potentials, L2P, the translation GEMMs on FP32 MFMA) with FP64 storage, coordinates and
-final sums.  Timed beside it: the mixed path with the full-precision operators (FP32 P2P and M2L, FP64
-pseudo-inverse solves and translations) and the plain FP64 path.  Errors are relative L2 against an FP64 direct
-sum on 2,000 seeded sample targets computed by the engine's direct-sum kernel.
+path of this workload: FP32 near-field P2P (Kahan-combined 64-source tiles) and FP32 M2L (transforms, and
+the Hadamard products on the f16 matrix cores with split-FP16 operands and FP32 accumulation), with the
+check-to-equivalent pseudo-inverse solves, the M2M / L2L translations, the operators and all storage in
+FP64; its pseudo-inverses drop the singular values below 4e-8 max(S).  Timed beside it: the FLOAT mode (the
+counterpart of the reference's -DFLOAT build: every evaluation stage in FP32, translation GEMMs on FP32
+MFMA) and the plain FP64 path (untruncated operators).  Errors are relative L2 against an FP64 direct sum
+on 2,000 seeded sample targets computed by the engine's direct-sum kernel.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--n 2000000] [--no-cpu-baseline]
 
@@ -40,11 +41,15 @@ PEAK = {"fp32_vector": 157.3e12, "fp64_vector": 78.6e12, "fp32_mfma": 157.3e12, 
         "fp16_mfma": 2.5e15, "hbm": 8.0e12}
 
 
-DTYPE = {"float": "f32 arithmetic in every evaluation stage (f16-split Hadamard, f32 MFMA translation GEMMs), "
+DTYPE = {"mixed": "f32 P2P (Kahan across tiles) and M2L (f32 transforms; Hadamard on f16 matrix cores, split-f16 "
+                  "operands, f32 accumulation) + f64 pseudo-inverse solves (truncated at 4e-8 max(S)), f64 "
+                  "M2M/L2L, operators and storage",
+         "float": "f32 arithmetic in every evaluation stage (f16-split Hadamard, f32 MFMA translation GEMMs), "
                   "f64 storage and setup; pseudo-inverses truncated at 4e-8 max(S)",
-         "mixed": "f32 P2P/M2L (f16-split Hadamard) + f64 pinv solves and translations",
          "fp64": "f64"}
-PATHS = (("float", 2), ("mixed", 1), ("fp64", 0))  # abi.FLOAT, abi.MIXED, abi.FP64; the first is the headline
+PRECISION_LABEL = {"mixed": "mixed: f32 P2P/M2L + f64 pseudo-inverse solves",
+                   "float": "float: every evaluation stage f32", "fp64": "fp64"}
+PATHS = (("mixed", 1), ("float", 2), ("fp64", 0))  # abi.MIXED, abi.FLOAT, abi.FP64; the first is the headline
 
 
 def compare_fields(res, head):
@@ -89,20 +94,26 @@ def setup_problem(n, p, ncrit, seed):
     ops = fmm.operators()
     qs = fmm.sorted_charges(tree)
     t_flat = time.perf_counter() - t0
-    return dict(x=x, q=q, flat=flat, ops=ops, ops_float=float_operators(fmm), qs=qs, depth=fmm.depth,
+    return dict(x=x, q=q, flat=flat, ops=ops, ops_trunc=truncated_operators(fmm), qs=qs, depth=fmm.depth,
                 t_setup=t_setup, t_flat=t_flat)
 
 
-def float_operators(fmm):
-    """Operators of the FLOAT mode: the same precompute with the pseudo-inverses truncated at 4e-8 max(S)."""
+def truncated_operators(fmm):
+    """Operators of the mixed and FLOAT modes: the same precompute with the pseudo-inverses truncated at
+    4e-8 max(S)."""
     from exafmm import abi
     save = fmm.precision
-    fmm.precision = abi.FLOAT
+    fmm.precision = abi.MIXED
     fmm.precompute()
     ops = fmm.operators()
     fmm.precision = save
     fmm.precompute()
     return ops
+
+
+def operators_for(prob, precision):
+    from exafmm import abi
+    return prob["ops"] if precision == abi.FP64 else prob["ops_trunc"]
 
 
 def p2p_pairs(flat):
@@ -116,7 +127,7 @@ def p2p_pairs(flat):
 def run_path(prob, p, precision, steps, warmup):
     from exafmm import abi
     eng = abi.Engine(abi.LAPLACE, p, 0.0, precision, 0)
-    eng.set_operators(prob["ops_float"] if precision == abi.FLOAT else prob["ops"])
+    eng.set_operators(operators_for(prob, precision))
     eng.set_tree(prob["flat"])
     eng.set_charges(prob["qs"])
     eng.synchronize()
@@ -152,6 +163,14 @@ def sample_error(prob, values, nsample, seed):
     return ep, eg
 
 
+def omp_threads():
+    """Threads the OpenMP oracle runs with: OMP_NUM_THREADS when set, else the CPUs this process may run on
+    (its affinity mask, not the machine's core count)."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    avail = len(os.sched_getaffinity(0))
+    return min(int(env), avail) if env.isdigit() and int(env) > 0 else avail
+
+
 def cpu_baseline(p, ncrit, n_sample):
     """The oracle (plain C + OpenMP restatement of the reference evaluation)
     timed on the host cores on a bounded sample: the same workload at a
@@ -164,7 +183,7 @@ def cpu_baseline(p, ncrit, n_sample):
     t0 = time.perf_counter()
     orc.evaluate()
     dt = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cores = omp_threads()
     return {"value": n_sample / dt, "unit": "particles/s", "cores": cores, "kind": "port",
             "sample": "oracle full upward+downward pass, N=%d uniform (same p=%d, ncrit=%d), %.1f s"
                       % (n_sample, p, ncrit, dt)}
@@ -236,14 +255,14 @@ def setup_generic(kid, n, p, ncrit, wavek, dist, charges, seed):
     flat = fmm.flat_tree(tree)
     ops = fmm.operators()
     qs = fmm.sorted_charges(tree)
-    return dict(kid=kid, p=p, wavek=wavek, flat=flat, ops=ops, ops_float=float_operators(fmm), qs=qs,
+    return dict(kid=kid, p=p, wavek=wavek, flat=flat, ops=ops, ops_trunc=truncated_operators(fmm), qs=qs,
                 depth=fmm.depth, n=n, t_setup=time.perf_counter() - t0)
 
 
 def engine_for(prob, precision):
     from exafmm import abi
     eng = abi.Engine(prob["kid"], prob["p"], prob["wavek"], precision, 0)
-    eng.set_operators(prob["ops_float"] if precision == abi.FLOAT else prob["ops"])
+    eng.set_operators(operators_for(prob, precision))
     eng.set_tree(prob["flat"])
     eng.set_charges(prob["qs"])
     return eng
@@ -287,12 +306,12 @@ def run_single_workload(a, name):
         eng.close()
         res[pname] = dict(ms=1e3 * dt, value=n / dt, err=err, kernels={k: round(v, 4) for k, v in kt.items()})
         log("%s %s: %.3f ms/step err phi %.2e grad %.2e" % (name, pname, 1e3 * dt, err[0], err[1]))
-    m = res["float"]
+    m = res["mixed"]
     line = {
         "metric": "particles/s, %s FMM evaluation" % name, "value": m["value"], "unit": "particles/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": m["ms"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": DTYPE["float"] + (" (complex; complex GEMMs f64)" if kid == 1 else ""),
+        "dtype": DTYPE["mixed"] + (" (complex)" if kid == 1 else ""),
         "data": "synthetic (seeded %s, charges %s)" % (dist, charges),
         "config": {"workload": "%s_N%d_p%d_ncrit%d" % (name, n, p, ncrit), "kernel": ["laplace", "helmholtz",
                    "modified_helmholtz"][kid], "wavek": wavek, "N": n, "p": p, "ncrit": ncrit,
@@ -300,7 +319,8 @@ def run_single_workload(a, name):
         "err_phi": m["err"][0], "err_grad": m["err"][1], "kernels_ms": m["kernels"],
         "setup_s": round(prob["t_setup"], 2), "roofline": None, "cpu_baseline": None,
     }
-    line.update(compare_fields(res, "float"))
+    line["config"]["precision"] = PRECISION_LABEL["mixed"]
+    line.update(compare_fields(res, "mixed"))
     return line
 
 
@@ -338,11 +358,11 @@ def run_batched(a, nprob=64):
                           err=(max(x[0] for x in errs), max(x[1] for x in errs)))
         log("batched %s: %.3f ms per batch, max err phi %.2e grad %.2e" % (pname, 1e3 * dt, res[pname]["err"][0],
                                                                            res[pname]["err"][1]))
-    m = res["float"]
+    m = res["mixed"]
     line = {
         "metric": "particles/s, laplace_batched FMM evaluation (aggregate)", "value": m["value"],
         "unit": "particles/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": m["ms"],
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": DTYPE["float"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": DTYPE["mixed"],
         "data": "synthetic (seeded Plummer, scale radius 1, truncated at r=10; charges 1/N)",
         "config": {"workload": "laplace_batched_%dproblems_p8_ncrit64" % nprob, "kernel": "laplace",
                    "problems": nprob, "N_total": int(ns.sum()), "N_min": int(ns.min()), "N_max": int(ns.max()),
@@ -350,7 +370,7 @@ def run_batched(a, nprob=64):
         "err_phi": m["err"][0], "err_grad": m["err"][1],
         "setup_s": round(t_setup, 2), "roofline": None, "cpu_baseline": None,
     }
-    line.update(compare_fields(res, "float"))
+    line.update(compare_fields(res, "mixed"))
     return line
 
 
@@ -413,7 +433,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
-    ap.add_argument("--mixed-only", action="store_true", help="time the headline mixed-precision (FLOAT) path only")
+    ap.add_argument("--mixed-only", action="store_true", help="time the headline mixed-precision path only")
     ap.add_argument("--workload", default="laplace_uniform",
                     choices=["laplace_uniform", "laplace_batched"] + sorted(WORKLOADS))
     a = ap.parse_args()
@@ -492,8 +512,7 @@ def main():
         "config": {"workload": "laplace_fmm_N%d_p%d_ncrit%d_uniform" % (a.n, a.p, a.ncrit), "kernel": "laplace",
                    "N": a.n, "p": a.p, "ncrit": a.ncrit, "depth": prob["depth"], "nodes": int(flat["nnodes"]),
                    "p2p_pairs": npairs, "m2l_pairs": m2l_pairs,
-                   "precision": {"float": "mixed: FLOAT mode (f32 evaluation, f64 storage)", "mixed": "mixed",
-                                 "fp64": "fp64"}[hname]},
+                   "precision": PRECISION_LABEL[hname]},
         "err_phi": head["err"][0],
         "err_grad": head["err"][1],
         "kernels_ms": kt,

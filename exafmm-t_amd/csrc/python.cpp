@@ -272,13 +272,15 @@ void bind_fmm(py::module& m, py::class_<F>& c) {
       .def_property_readonly("x0", [](const F& f) { return std::vector<double>{f.x0[0], f.x0[1], f.x0[2]}; })
       .def_readwrite("filename", &F::filename)
       .def_readwrite("precision", &F::precision,
-                     "0: FP64 device path, 1: mixed (FP32 P2P and M2L), 2: FLOAT (all evaluation stages FP32, "
-                     "pseudo-inverses truncated at 4e-8; set before setup)")
+                     "0: FP64 device path; 1: mixed (FP32 P2P and M2L, FP64 pseudo-inverse solves and "
+                     "translations); 2: FLOAT (all evaluation stages FP32).  1 and 2 truncate the pseudo-inverses "
+                     "at 4e-8 max(S); set before setup or call precompute()")
       .def_readwrite("device", &F::device)
       .def("flat_tree", [](F& f, Tree<T>& t) { return flat_tree<T, F>(f, t); })
       .def("operators", [](F& f) { return operators<T, F>(f); })
       .def("precompute", &F::precompute,
-           "recompute the operators for the current precision (FLOAT truncates the pseudo-inverses at 4e-8)")
+           "recompute the operators for the current precision (mixed and FLOAT truncate the pseudo-inverses "
+           "at 4e-8 max(S))")
       .def("sorted_charges", [](F& f, Tree<T>& t) { return sorted_charges<T, F>(f, t); })
       .def("sync_equiv_to_host", [](F& f, Tree<T>& t) { f.sync_equiv_to_host(t.nodes); })
       .def("release_device", &F::release_device);

@@ -38,16 +38,32 @@ def points(n, dist, rng):
     raise ValueError(dist)
 
 
-def make_problem(kernel, n, p, ncrit=64, dist="cube", seed=0, wavek=None, precision=0, depth=None):
+def charges(kind, n, rng, cplx):
+    """Charge distributions of the workloads: "centered" U[-0.5,0.5), "u01" U[0,1), "u11" U[-1,1), "normal"
+    N(0,1); complex kernels get an independent imaginary part of the same distribution."""
+    def draw():
+        if kind == "centered":
+            return rng.random(n) - 0.5
+        if kind == "u01":
+            return rng.random(n)
+        if kind == "u11":
+            return rng.uniform(-1.0, 1.0, n)
+        if kind == "normal":
+            return rng.normal(size=n)
+        raise ValueError(kind)
+    q = draw()
+    return q + 1j * draw() if cplx else q
+
+
+def make_problem(kernel, n, p, ncrit=64, dist="cube", seed=0, wavek=None, precision=0, depth=None,
+                 charge="centered"):
     """depth=None: adaptive tree (setup); depth=d: the non-adaptive tree of that
     depth (setup_non_adaptive, the reference's NON_ADAPTIVE build)."""
     mod, cls, kid, wk_default = KERNELS[kernel]
     wk = wk_default if wavek is None else wavek
     rng = np.random.default_rng(seed)
     x = points(n, dist, rng)
-    q = rng.random(n) - 0.5
-    if kid == 1:
-        q = q + 1j * (rng.random(n) - 0.5)
+    q = charges(charge, n, rng, kid == 1)
     src = mod.init_sources(x, q)
     trg = mod.init_targets(x)
     Cls = getattr(mod, cls)
@@ -57,6 +73,19 @@ def make_problem(kernel, n, p, ncrit=64, dist="cube", seed=0, wavek=None, precis
     tree = mod.setup(src, trg, fmm) if depth is None else mod.setup_non_adaptive(src, trg, fmm, depth)
     return dict(mod=mod, fmm=fmm, tree=tree, kid=kid, wavek=0.0 if wk is None else wk, x=x, q=q, p=p,
                 flat=fmm.flat_tree(tree), ops=fmm.operators(), qs=fmm.sorted_charges(tree))
+
+
+def truncated_ops(pr):
+    """Operators of the mixed and FLOAT modes for problem `pr`: the same precompute with the pseudo-inverses
+    truncated at 4e-8 max(S) (the operators of pr itself are kept)."""
+    fmm = pr["fmm"]
+    save = fmm.precision
+    fmm.precision = 1
+    fmm.precompute()
+    ops = fmm.operators()
+    fmm.precision = save
+    fmm.precompute()
+    return ops
 
 
 def rel_err(a, b):

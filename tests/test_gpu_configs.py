@@ -1,0 +1,128 @@
+"""GPU parity at the parameters of the benchmark workloads, scaled to sizes the
+oracle finishes in seconds: the kernel, point distribution, charge
+distribution, expansion order, ncrit and wave number of each workload.
+
+* Laplace, uniform cube, charges U[-1,1), p = 10, ncrit = 128 (the N = 2M
+  headline workload), N = 60k: FP64 after every operator and for full passes
+  within 1e-10 of the oracle; the mixed path and the FLOAT mode within 1e-5 of
+  the oracle on the same (truncated) operators, for potential and gradient,
+  and within 1e-5 of the FP64 direct sum.
+* Laplace, uniform cube, charges U[0,1), p = 8, ncrit = 64 at its full size
+  N = 100k: FP64 within 1e-10 of the oracle; FMM error against the direct sum.
+* Modified Helmholtz k = 5, unit sphere (normalised Gaussian samples), charges
+  N(0,1), p = 8, ncrit = 64, N = 50k, an adaptive tree whose M2P and P2L lists
+  are non-empty (asserted): FP64 per operator and full passes within 1e-10;
+  mixed path within 1e-5.
+* Helmholtz k = 20 (real), uniform cube, complex charges U[-1,1) + i U[-1,1),
+  p = 10, ncrit = 128, N = 30k: FP64 full passes within 1e-10; the complex64
+  mixed path within 1e-5.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from exafmm import abi
+from fmm_cases import make_problem, rel_err, truncated_ops
+from test_gpu_parity import FP64_TOL, MIXED_TOL, _engine, check_stages_fp64
+
+pytestmark = pytest.mark.gpu
+
+
+def lists(pr):
+    f = pr["flat"]
+    return dict(depth=int(f["depth"]), nodes=int(f["nnodes"]), m2p=int(f["m2p_offset"][-1]),
+                p2l=int(f["p2l_offset"][-1]), m2l=int(np.sum(f["m2l_source"] >= 0)),
+                p2p=int(f["p2p_offset"][-1]))
+
+
+def direct_sample(pr, nsample, seed):
+    tc = pr["flat"]["trg_coord"].reshape(-1, 3)
+    rng = np.random.default_rng(seed)
+    idx = np.sort(rng.choice(tc.shape[0], size=min(nsample, tc.shape[0]), replace=False))
+    d = O.direct(pr["kid"], pr["flat"]["src_coord"].reshape(-1, 3), pr["qs"], tc[idx], pr["wavek"])
+    return idx, d
+
+
+def errs(v, r):
+    return rel_err(v[:, 0], r[:, 0]), rel_err(v[:, 1:], r[:, 1:])
+
+
+def check_truncated_paths(pr, precisions, idx, d, ref64):
+    """Mixed / FLOAT engine against the oracle on the truncated operators (1e-5,
+    potential and gradient) and against the FP64 direct sum (1e-5, or twice the
+    FP64 FMM's own error where that is larger)."""
+    ops_t = truncated_ops(pr)
+    orc = O.Oracle(pr["kid"], pr["p"], pr["flat"], ops_t, pr["wavek"])
+    orc.set_charges(pr["qs"])
+    ref_t = orc.evaluate()
+    e64 = errs(ref64[idx], d)
+    for prec in precisions:
+        eng = _engine(pr, prec, ops_t)
+        eng.upward_pass()
+        eng.downward_pass()
+        v = eng.values()
+        eng.close()
+        eo, ed = errs(v, ref_t), errs(v[idx], d)
+        print("precision %d vs oracle phi %.2e grad %.2e; vs direct phi %.2e grad %.2e (fp64 fmm %.2e %.2e)"
+              % (prec, eo[0], eo[1], ed[0], ed[1], e64[0], e64[1]), flush=True)
+        assert eo[0] < MIXED_TOL and eo[1] < MIXED_TOL, (prec, eo)
+        assert ed[0] < max(2 * e64[0], MIXED_TOL) and ed[1] < max(2 * e64[1], MIXED_TOL), (prec, ed, e64)
+
+
+def test_laplace_headline_parameters(require_gpu):
+    pr = make_problem("laplace", 60000, 10, ncrit=128, dist="cube", seed=40, charge="u11")
+    print("laplace p=10 ncrit=128 uniform N=60000", lists(pr), flush=True)
+    ref = check_stages_fp64(pr)
+    idx, d = direct_sample(pr, 2000, 123)
+    e64 = errs(ref[idx], d)
+    print("fp64 fmm vs direct phi %.2e grad %.2e" % e64, flush=True)
+    assert e64[0] < 1e-6 and e64[1] < 1e-5
+    check_truncated_paths(pr, (abi.MIXED, abi.FLOAT), idx, d, ref)
+
+
+def test_laplace_p8_full_size(require_gpu):
+    pr = make_problem("laplace", 100000, 8, ncrit=64, dist="cube", seed=41, charge="u01")
+    print("laplace p=8 ncrit=64 uniform N=100000", lists(pr), flush=True)
+    orc = O.Oracle(pr["kid"], 8, pr["flat"], pr["ops"], pr["wavek"])
+    orc.set_charges(pr["qs"])
+    ref = orc.evaluate()
+    eng = _engine(pr, abi.FP64)
+    eng.upward_pass()
+    eng.downward_pass()
+    v = eng.values()
+    eng.close()
+    e = errs(v, ref)
+    idx, d = direct_sample(pr, 1000, 7)
+    ed = errs(v[idx], d)
+    print("fp64 vs oracle phi %.2e grad %.2e; vs direct phi %.2e grad %.2e" % (e + ed), flush=True)
+    assert e[0] < FP64_TOL and e[1] < FP64_TOL
+    assert ed[0] < 1e-6 and ed[1] < 1e-5
+
+
+def test_yukawa_sphere_adaptive(require_gpu):
+    pr = make_problem("modified_helmholtz", 50000, 8, ncrit=64, dist="sphere", seed=3, wavek=5.0, charge="normal")
+    ls = lists(pr)
+    print("yukawa k=5 sphere p=8 ncrit=64 N=50000", ls, flush=True)
+    # the adaptive corrections must be exercised (reference build_list.h:16-114)
+    assert ls["m2p"] > 0 and ls["p2l"] > 0, ls
+    ref = check_stages_fp64(pr)
+    idx, d = direct_sample(pr, 1000, 5)
+    check_truncated_paths(pr, (abi.MIXED,), idx, d, ref)
+
+
+def test_helmholtz_k20_parameters(require_gpu):
+    pr = make_problem("helmholtz", 30000, 10, ncrit=128, dist="cube", seed=42, wavek=20.0 + 0.0j, charge="u11")
+    print("helmholtz k=20 p=10 ncrit=128 uniform N=30000", lists(pr), flush=True)
+    orc = O.Oracle(pr["kid"], 10, pr["flat"], pr["ops"], pr["wavek"])
+    orc.set_charges(pr["qs"])
+    ref = orc.evaluate()
+    eng = _engine(pr, abi.FP64)
+    for _ in range(2):
+        eng.upward_pass()
+        eng.downward_pass()
+        e = errs(eng.values(), ref)
+        print("fp64 vs oracle phi %.2e grad %.2e" % e, flush=True)
+        assert e[0] < FP64_TOL and e[1] < FP64_TOL
+    eng.close()
+    idx, d = direct_sample(pr, 1000, 9)
+    check_truncated_paths(pr, (abi.MIXED,), idx, d, ref)

@@ -227,6 +227,32 @@ def test_precompute_cache_reference_layout(tmp_path):
     assert os.path.getsize(fn) == (2 * 8 + 4) * ns * ns * 8 + 26 * nf * 128 * 8 + 8
 
 
+def test_precompute_cache_rejects_other_wave_number(tmp_path):
+    # The reference layout does not record the wave number (fmm.h:103-138): a
+    # file written for k = 5 has the size a k = 2 run expects.  The loaded
+    # pseudo-inverses are checked against this kernel's check matrix, so the
+    # k = 2 run recomputes instead of evaluating with k = 5 operators.
+    Y = exafmm.modified_helmholtz
+    x = np.random.default_rng(1).random((1500, 3))
+    q = np.ones(1500)
+    fn = str(tmp_path / "yuk.dat")
+    f5 = Y.ModifiedHelmholtzFmm(4, 64, 5.0, fn)
+    Y.setup(Y.init_sources(x, q), Y.init_targets(x), f5)
+    size5 = os.path.getsize(fn)
+    f2 = Y.ModifiedHelmholtzFmm(4, 64, 2.0, fn)
+    Y.setup(Y.init_sources(x, q), Y.init_targets(x), f2)
+    assert os.path.getsize(fn) == size5
+    fresh = Y.ModifiedHelmholtzFmm(4, 64, 2.0, str(tmp_path / "fresh.dat"))
+    Y.setup(Y.init_sources(x, q), Y.init_targets(x), fresh)
+    for k in ("uc2e_v", "m2m", "m2l"):
+        assert rel_err(f2.operators()[k], fresh.operators()[k]) < 1e-12, k
+        assert rel_err(f2.operators()[k], f5.operators()[k]) > 1e-3, k
+    # the file now holds k = 2 operators: a k = 2 run loads them unchanged
+    f2b = Y.ModifiedHelmholtzFmm(4, 64, 2.0, fn)
+    Y.setup(Y.init_sources(x, q), Y.init_targets(x), f2b)
+    assert np.array_equal(f2b.operators()["m2l"], f2.operators()["m2l"])
+
+
 def test_python_api_errors():
     L = exafmm.laplace
     with pytest.raises(RuntimeError):
