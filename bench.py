@@ -171,22 +171,24 @@ def omp_threads():
     return min(int(env), avail) if env.isdigit() and int(env) > 0 else avail
 
 
-def cpu_baseline(p, ncrit, n_sample):
-    """The oracle (plain C + OpenMP restatement of the reference evaluation)
-    timed on the host cores on a bounded sample: the same workload at a
-    smaller N (same distribution, p, ncrit)."""
+def cpu_baseline(prob, p, ncrit, n):
+    """The CPU baseline: the oracle's FFT-based evaluation (oracle/fast.c: the reference's algorithm at its
+    cost -- FFT transforms, blocked Hadamard, GEMM translations, vectorised P2P; FP64, OpenMP on the host
+    cores) on the same tree, charges and FP64 operators as the GPU paths, at the full N: one untimed and
+    one timed evaluation."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    prob = setup_problem(n_sample, p, ncrit, seed=1)
     orc = O.Oracle(O.LAPLACE, p, prob["flat"], prob["ops"])
     orc.set_charges(prob["qs"])
+    orc.evaluate_fast()
     t0 = time.perf_counter()
-    orc.evaluate()
+    orc.evaluate_fast()
     dt = time.perf_counter() - t0
+    st = {k: round(v, 3) for k, v in orc.fast_stage_s.items()}
     cores = omp_threads()
-    return {"value": n_sample / dt, "unit": "particles/s", "cores": cores, "kind": "port",
-            "sample": "oracle full upward+downward pass, N=%d uniform (same p=%d, ncrit=%d), %.1f s"
-                      % (n_sample, p, ncrit, dt)}
+    return {"value": n / dt, "unit": "particles/s", "cores": cores, "kind": "port",
+            "sample": "oracle/fast.c full upward+downward pass of the same workload (N=%d uniform, p=%d, "
+                      "ncrit=%d, FP64), %.2f s; stages (s) %s" % (n, p, ncrit, dt, json.dumps(st))}
 
 
 def measured_traffic(dom, mixed, a):
@@ -324,40 +326,56 @@ def run_single_workload(a, name):
     return line
 
 
-def run_batched(a, nprob=64):
-    """64 independent Laplace problems; each evaluation captured in a HIP graph, all graphs replayed back to
-    back (each context has its own streams, so small problems overlap)."""
-    from exafmm import abi
-    rng = np.random.default_rng(5)
+def plummer_batch(nprob, seed=5):
+    """The batched workload: nprob Plummer problems (scale radius 1, radius truncated at 10), N log-uniform in
+    [10,000, 500,000], charges 1/N, targets = sources."""
+    rng = np.random.default_rng(seed)
     ns = np.exp(rng.uniform(np.log(1e4), np.log(5e5), nprob)).astype(int)
+    probs = []
+    for i, n in enumerate(ns):
+        x = make_points("plummer", int(n), np.random.default_rng(100 + i))
+        probs.append((x, np.full(int(n), 1.0 / float(n)), x))
+    return ns, probs
+
+
+def run_batched(a, nprob=64):
+    """nprob independent Laplace problems evaluated as one forest (exafmm.batch): one operator set, one launch
+    per operator for all problems, the launch shapes picked by the engine's autotuner, one captured graph
+    replayed per step.  Per-problem errors against the FP64 direct sum on 200 sampled targets each."""
+    from exafmm import abi
+    from exafmm.batch import LaplaceBatch, LaplaceForest
+    ns, probs = plummer_batch(nprob)
     t0 = time.perf_counter()
-    probs = [setup_generic(0, int(n), 8, 64, 0.0, "plummer", 1.0 / float(n), seed=100 + i) for i, n in enumerate(ns)]
+    forest = LaplaceForest(probs, p=8, ncrit=64)
     t_setup = time.perf_counter() - t0
-    log("batched setup %.1fs, %d problems, %d particles" % (t_setup, nprob, int(ns.sum())))
+    log("batched setup %.1fs, %d problems, %d particles, forest %d nodes depth %d"
+        % (t_setup, nprob, int(ns.sum()), forest.flat["nnodes"], forest.depth))
+    fname = os.path.join(tempfile.gettempdir(), "bench_batch_p8.dat")
     res = {}
     for pname, prec in PATHS:
-        engs = [engine_for(pr, prec) for pr in probs]
-        for e in engs:
-            e.graph_capture()
+        b = LaplaceBatch(forest, precision=prec, filename=fname, autotune=True, graph=True)
         for _ in range(max(1, a.warmup)):
-            for e in engs:
-                e.graph_launch()
-            for e in engs:
-                e.synchronize()
+            b.run()
+        b.engine.synchronize()
         t0 = time.perf_counter()
         for _ in range(a.steps):
-            for e in engs:
-                e.graph_launch()
-            for e in engs:
-                e.synchronize()
+            b.run()
+        b.engine.synchronize()
         dt = (time.perf_counter() - t0) / a.steps
-        errs = [sample_error_generic(pr, e.values(), 200, seed=7) for pr, e in zip(probs, engs)]
-        for e in engs:
-            e.close()
-        res[pname] = dict(ms=1e3 * dt, value=float(ns.sum()) / dt,
+        vals = b.values()
+        errs = []
+        for i, (x, q, _) in enumerate(probs):
+            idx = np.sort(np.random.default_rng(7 + i).choice(x.shape[0], 200, replace=False))
+            ref = abi.direct(abi.LAPLACE, x, q, x[idx], 0.0, abi.FP64, True)
+            v = vals[i][idx]
+            errs.append((float(np.linalg.norm(v[:, 0] - ref[:, 0]) / np.linalg.norm(ref[:, 0])),
+                         float(np.linalg.norm(v[:, 1:] - ref[:, 1:]) / np.linalg.norm(ref[:, 1:]))))
+        tuning = b.tuning
+        b.close()
+        res[pname] = dict(ms=1e3 * dt, value=float(ns.sum()) / dt, tuning=tuning,
                           err=(max(x[0] for x in errs), max(x[1] for x in errs)))
-        log("batched %s: %.3f ms per batch, max err phi %.2e grad %.2e" % (pname, 1e3 * dt, res[pname]["err"][0],
-                                                                           res[pname]["err"][1]))
+        log("batched %s: %.3f ms per batch, max err phi %.2e grad %.2e, tuning %s"
+            % (pname, 1e3 * dt, res[pname]["err"][0], res[pname]["err"][1], tuning))
     m = res["mixed"]
     line = {
         "metric": "particles/s, laplace_batched FMM evaluation (aggregate)", "value": m["value"],
@@ -366,7 +384,9 @@ def run_batched(a, nprob=64):
         "data": "synthetic (seeded Plummer, scale radius 1, truncated at r=10; charges 1/N)",
         "config": {"workload": "laplace_batched_%dproblems_p8_ncrit64" % nprob, "kernel": "laplace",
                    "problems": nprob, "N_total": int(ns.sum()), "N_min": int(ns.min()), "N_max": int(ns.max()),
-                   "p": 8, "ncrit": 64, "graph": "one captured evaluation per problem"},
+                   "p": 8, "ncrit": 64, "forest_nodes": int(forest.flat["nnodes"]), "depth": forest.depth,
+                   "graph": "one captured evaluation of the whole forest", "tuning": m["tuning"],
+                   "precision": PRECISION_LABEL["mixed"]},
         "err_phi": m["err"][0], "err_grad": m["err"][1],
         "setup_s": round(t_setup, 2), "roofline": None, "cpu_baseline": None,
     }
@@ -430,7 +450,6 @@ def main():
     ap.add_argument("--p", type=int, default=10)
     ap.add_argument("--ncrit", type=int, default=128)
     ap.add_argument("--nsample", type=int, default=2000)
-    ap.add_argument("--cpu-sample", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--mixed-only", action="store_true", help="time the headline mixed-precision path only")
@@ -521,8 +540,8 @@ def main():
     }
     line.update(compare_fields(res, hname))
     if not a.no_cpu_baseline:
-        log("cpu baseline (oracle) on N=%d" % a.cpu_sample)
-        line["cpu_baseline"] = cpu_baseline(a.p, a.ncrit, a.cpu_sample)
+        log("cpu baseline (oracle/fast.c) on the same workload, N=%d" % a.n)
+        line["cpu_baseline"] = cpu_baseline(prob, a.p, a.ncrit, a.n)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)

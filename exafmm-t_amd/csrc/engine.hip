@@ -71,7 +71,9 @@ struct xfmm_context {
   // (transforms + Hadamard), and the surface evaluations P2M/L2P/M2P/P2L
   bool f32_p2p = false, f32_m2l = false, f32_surf = false, f32_m2l_fft = false, f32_far = false;
   bool f16_had = false;  // split-FP16 Hadamard (needs f32_m2l)
-  bool f32_gemm = false;  // FP32 MFMA translation GEMMs (FLOAT mode: float-truncated pseudo-inverses)
+  // FP32 MFMA GEMMs: the pseudo-inverse solves of P2M / L2P (FLOAT mode only)
+  // and the M2M / L2L translations (both need the float-truncated operators)
+  bool f32_pinv = false, f32_trans = false;
   // dn and trg_value are zero (cleared by the last upward pass or clear_values
   // and not written since), so the downward pass need not clear them itself
   bool down_clean = false;
@@ -102,7 +104,7 @@ struct xfmm_context {
   // numerical rank of the upward / downward pseudo-inverse per operator level
   // (uc2e_uT / dc2e_uT hold only the first r_up / r_dn columns, row stride r)
   std::vector<int> r_up, r_dn;
-  DevBuf<float> uc2e_uT32, uc2e_vT32, dc2e_uT32, dc2e_vT32, m2mcat32, l2lcat32;  // f32_gemm (real kernels)
+  DevBuf<float> uc2e_uT32, uc2e_vT32, dc2e_uT32, dc2e_vT32, m2mcat32, l2lcat32;  // f32_pinv / f32_trans (real kernels)
   DevBuf<double> m2l64;  // M2L operators in the expanded MFMA form, [L][nfreq][26][64][4]
   DevBuf<float> m2l32;   // FP32 form, or the split-FP16 form of the same size when f16_had
   DevBuf<float> m2l_hscale;  // f16_had: operator scale per [L][nfreq]
@@ -138,6 +140,13 @@ struct xfmm_context {
   DevBuf<double> up, dn, trg_value, check, tmp, gemm_scratch, near;
   DevBuf<double> src_rel, src_pack, surf_tab;
   DevBuf<float> surf_tab32;  // P2P sources relative to their leaf centre; packed (x, y, z, q)
+
+  // per-target output factors (phi, gradient) applied at the end of each
+  // downward pass (xfmm_set_value_scale); empty = none
+  DevBuf<double> vscale;
+  // tuning (xfmm_set_tuning / xfmm_autotune)
+  int tune_kch = HAD_KCH;  // Hadamard frequencies per workgroup
+  int tune_split = 32;     // real-kernel P2P / L2P: largest tail tile whose sources are split over lanes
 
   // ---- timing
   hipEvent_t ev[24] = {};
@@ -180,6 +189,7 @@ struct xfmm_context {
     t.up = up.ptr;
     t.dn = dn.ptr;
     t.trg_value = trg_value.ptr;
+    t.p2p_split = tune_split;
     return t;
   }
   M2LDev m2l_dev() const {
@@ -302,8 +312,8 @@ void check_ready(xfmm_context* c) {
 
 // B operand of a translation GEMM: the FP32 copy for FP32 GEMMs of the real
 // kernels, else the FP64 operator; off = element offset.
-const void* gemm_op(const xfmm_context* c, const DevBuf<double>& d, const DevBuf<float>& f, size_t off) {
-  if (c->f32_gemm && !c->cplx) return f.ptr + off;
+const void* gemm_op(bool f32, const xfmm_context* c, const DevBuf<double>& d, const DevBuf<float>& f, size_t off) {
+  if (f32 && !c->cplx) return f.ptr + off;
   return d.ptr + off;
 }
 
@@ -320,11 +330,11 @@ void op_P2M(xfmm_context* c) {
     // only the first r singular directions carry a nonzero S^+ entry: the
     // first product computes those r coefficients, the second sums over them
     const int r = c->r_up[L];
-    launch_gemm_rows(c->cplx, c->f32_gemm, e - b, r, c->nsurf, c->check.ptr, c->leaf_slot_rows.ptr + b,
-                     gemm_op(c, c->uc2e_uT, c->uc2e_uT32, L * ns2), c->tmp.ptr, c->leaf_slot_rows.ptr + b, 1.0, 0,
-                     c->stream);
-    launch_gemm_rows(c->cplx, c->f32_gemm, e - b, c->nsurf, r, c->tmp.ptr, c->leaf_slot_rows.ptr + b,
-                     gemm_op(c, c->uc2e_vT, c->uc2e_vT32, L * ns2), c->up.ptr, c->leaf_node_rows.ptr + b,
+    launch_gemm_rows(c->cplx, c->f32_pinv, e - b, r, c->nsurf, c->check.ptr, c->leaf_slot_rows.ptr + b,
+                     gemm_op(c->f32_pinv, c, c->uc2e_uT, c->uc2e_uT32, L * ns2), c->tmp.ptr, c->leaf_slot_rows.ptr + b,
+                     1.0, 0, c->stream);
+    launch_gemm_rows(c->cplx, c->f32_pinv, e - b, c->nsurf, r, c->tmp.ptr, c->leaf_slot_rows.ptr + b,
+                     gemm_op(c->f32_pinv, c, c->uc2e_vT, c->uc2e_vT32, L * ns2), c->up.ptr, c->leaf_node_rows.ptr + b,
                      c->leaf_scale(l), 0, c->stream);
   }
 }
@@ -336,8 +346,8 @@ void op_M2M(xfmm_context* c) {
     if (e <= b) continue;
     const int L = c->op_level(l);
     // parent row = [child_0 .. child_7] (contiguous, K = 8 nsurf) x M2Mcat
-    launch_gemm_rows(c->cplx, c->f32_gemm, e - b, c->nsurf, 8 * c->nsurf, c->up.ptr, c->nl_child_rows.ptr + b,
-                     gemm_op(c, c->m2mcat, c->m2mcat32, L * blk), c->up.ptr, c->nl_rows.ptr + b, 1.0, 1, c->stream,
+    launch_gemm_rows(c->cplx, c->f32_trans, e - b, c->nsurf, 8 * c->nsurf, c->up.ptr, c->nl_child_rows.ptr + b,
+                     gemm_op(c->f32_trans, c, c->m2mcat, c->m2mcat32, L * blk), c->up.ptr, c->nl_rows.ptr + b, 1.0, 1, c->stream,
                      c->gemm_scratch.ptr, c->gemm_scratch.n);
   }
 }
@@ -348,8 +358,8 @@ void op_L2L(xfmm_context* c) {
     const int b = c->nl_level_off[l], e = c->nl_level_off[l + 1];
     if (e <= b) continue;
     const int L = c->op_level(l);
-    launch_gemm_rows(c->cplx, c->f32_gemm, e - b, 8 * c->nsurf, c->nsurf, c->dn.ptr, c->nl_rows.ptr + b,
-                     gemm_op(c, c->l2lcat, c->l2lcat32, L * blk),
+    launch_gemm_rows(c->cplx, c->f32_trans, e - b, 8 * c->nsurf, c->nsurf, c->dn.ptr, c->nl_rows.ptr + b,
+                     gemm_op(c->f32_trans, c, c->l2lcat, c->l2lcat32, L * blk),
                      c->dn.ptr, c->nl_child_rows.ptr + b, 1.0, 1, c->stream, c->gemm_scratch.ptr,
                      c->gemm_scratch.n);
   }
@@ -362,11 +372,11 @@ void op_L2P(xfmm_context* c) {
     if (e <= b) continue;
     const int L = c->op_level(l);
     const int r = c->r_dn[L];
-    launch_gemm_rows(c->cplx, c->f32_gemm, e - b, r, c->nsurf, c->dn.ptr, c->leaf_node_rows.ptr + b,
-                     gemm_op(c, c->dc2e_uT, c->dc2e_uT32, L * ns2), c->tmp.ptr, c->leaf_slot_rows.ptr + b, 1.0, 0,
-                     c->stream);
-    launch_gemm_rows(c->cplx, c->f32_gemm, e - b, c->nsurf, r, c->tmp.ptr, c->leaf_slot_rows.ptr + b,
-                     gemm_op(c, c->dc2e_vT, c->dc2e_vT32, L * ns2), c->dn.ptr, c->leaf_node_rows.ptr + b,
+    launch_gemm_rows(c->cplx, c->f32_pinv, e - b, r, c->nsurf, c->dn.ptr, c->leaf_node_rows.ptr + b,
+                     gemm_op(c->f32_pinv, c, c->dc2e_uT, c->dc2e_uT32, L * ns2), c->tmp.ptr, c->leaf_slot_rows.ptr + b,
+                     1.0, 0, c->stream);
+    launch_gemm_rows(c->cplx, c->f32_pinv, e - b, c->nsurf, r, c->tmp.ptr, c->leaf_slot_rows.ptr + b,
+                     gemm_op(c->f32_pinv, c, c->dc2e_vT, c->dc2e_vT32, L * ns2), c->dn.ptr, c->leaf_node_rows.ptr + b,
                      c->leaf_scale(l), 0, c->stream);
   }
   DevTree t = c->dev_tree();
@@ -413,14 +423,8 @@ void launch_near(xfmm_context* c) {
   c->near_launched = true;
 }
 
-void op_M2L(xfmm_context* c, bool timed) {
-  if (c->n_m2l_trg == 0) return;
-  M2LDev m = c->m2l_dev();
-  DevTree t = c->dev_tree();
-  if (timed) record(c, 16);
-  if (m.amax) launch_zero(c->m2l_amax.ptr, 1, c->stream);
-  launch_fft_up(m, t, c->m2l_src_node.ptr, c->n_m2l_src, c->stream);
-  if (timed) record(c, 17);
+// hadamard_product of every operator level (after the forward transforms)
+void run_hadamards(xfmm_context* c, const M2LDev& m) {
   const size_t opsz = size_t(NM2L) * c->nfreq * 64 * 4;
   // one operator set (Laplace): a single launch over the blocks of all levels
   const int nlaunch = (c->nm2l == 1) ? 1 : c->depth;
@@ -437,8 +441,20 @@ void op_M2L(xfmm_context* c, bool timed) {
     hb.uni = c->m2l_blk_uni.ptr + size_t(b) * HAD_UCAP;
     hb.nunion = c->m2l_blk_nu.ptr + b;
     hb.loc = c->m2l_blk_loc.ptr + size_t(b) * NM2L * 64;
-    launch_hadamard(m, ops, c->f16_had ? c->m2l_hscale.ptr + size_t(L) * c->nfreq : nullptr, hb, c->stream);
+    launch_hadamard(m, ops, c->f16_had ? c->m2l_hscale.ptr + size_t(L) * c->nfreq : nullptr, hb, c->tune_kch,
+                    c->stream);
   }
+}
+
+void op_M2L(xfmm_context* c, bool timed) {
+  if (c->n_m2l_trg == 0) return;
+  M2LDev m = c->m2l_dev();
+  DevTree t = c->dev_tree();
+  if (timed) record(c, 16);
+  if (m.amax) launch_zero(c->m2l_amax.ptr, 1, c->stream);
+  launch_fft_up(m, t, c->m2l_src_node.ptr, c->n_m2l_src, c->stream);
+  if (timed) record(c, 17);
+  run_hadamards(c, m);
   if (timed) record(c, 18);
   launch_ifft_dn(m, t, c->m2l_trg_node.ptr, c->m2l_scale.ptr, c->n_m2l_trg, c->stream);
   if (timed) record(c, 19);
@@ -473,28 +489,29 @@ int xfmm_create(int kernel, int p, double wavek_re, double wavek_im, int precisi
   c->p = p;
   c->prec = precision;
   if (precision == XFMM_MIXED) {
-    // default mixed policy: FP32 near field and FP32 M2L (spectra, Hadamard
-    // and transform arithmetic), the Hadamard on split FP16 (two FP16 numbers
-    // per FP32 value, FP32 accumulation).  XFMM_MIXED_STAGES overrides it with
-    // a comma list of p2p, m2l (FP32 spectra + Hadamard), had16 (split-FP16
-    // Hadamard, with m2l), fft (FP32 transform arithmetic, else FP64), far
-    // (FP32 L2P / M2P evaluation), surf (FP32 check potentials of P2M /
-    // P2L) and gemm (FP32 translation GEMMs).  "far" is off by default: the FP32 M2L
-    // noise, amplified by the downward pseudo-inverse, gives the local
-    // equivalent densities large components that cancel only when the L2P
-    // sum is done in FP64.  "surf" loses all accuracy through the
-    // pseudo-inverses; both are kept for study.
+    // Mixed policy (the operators' pseudo-inverses truncated at 4e-8 max(S),
+    // which the host layer precomputes for this mode): FP32 near field, FP32
+    // M2L (transforms, spectra; the Hadamard on split FP16 = two FP16 numbers
+    // per FP32 value, FP32 accumulation), FP32 check potentials (P2M, P2L),
+    // FP32 L2P / M2P evaluation and FP32 M2M / L2L translations; the
+    // check-to-equivalent pseudo-inverse solves of P2M and L2P stay FP64.
+    // XFMM_MIXED_STAGES overrides the FP32 set with a comma list of p2p, m2l
+    // (FP32 spectra + Hadamard), had16 (split-FP16 Hadamard, with m2l), fft
+    // (FP32 transform arithmetic, else FP64), surf (check potentials), far
+    // (L2P / M2P evaluation), trans (M2M / L2L) and pinv (the solves).
     const char* env = std::getenv("XFMM_MIXED_STAGES");
-    const std::string st = env ? env : "p2p,m2l,had16,fft";
+    const std::string st = env ? env : "p2p,m2l,had16,fft,surf,far,trans";
     c->f32_p2p = st.find("p2p") != std::string::npos;
     c->f32_m2l = st.find("m2l") != std::string::npos;
     c->f32_surf = st.find("surf") != std::string::npos;
     c->f32_far = st.find("far") != std::string::npos;
     c->f32_m2l_fft = c->f32_m2l && st.find("fft") != std::string::npos;
     c->f16_had = c->f32_m2l && st.find("had16") != std::string::npos;
-    c->f32_gemm = st.find("gemm") != std::string::npos;
+    c->f32_trans = st.find("trans") != std::string::npos;
+    c->f32_pinv = st.find("pinv") != std::string::npos;
   } else if (precision == XFMM_FLOAT) {
-    c->f32_p2p = c->f32_m2l = c->f32_m2l_fft = c->f16_had = c->f32_surf = c->f32_far = c->f32_gemm = true;
+    c->f32_p2p = c->f32_m2l = c->f32_m2l_fft = c->f16_had = c->f32_surf = c->f32_far = true;
+    c->f32_pinv = c->f32_trans = true;
   }
   c->device = device;
   c->wk_re = wavek_re;
@@ -651,8 +668,9 @@ int xfmm_set_operators(xfmm_context* c, int nlevels, const double* uc2e_u, const
     const std::pair<DevBuf<double>*, DevBuf<float>*> f32ops[] = {
         {&c->uc2e_uT, &c->uc2e_uT32}, {&c->uc2e_vT, &c->uc2e_vT32}, {&c->dc2e_uT, &c->dc2e_uT32},
         {&c->dc2e_vT, &c->dc2e_vT32}, {&c->m2mcat, &c->m2mcat32},   {&c->l2lcat, &c->l2lcat32}};
-    for (const auto& pr : f32ops) {
-      if (c->f32_gemm && !c->cplx) {
+    for (int i = 0; i < 6; i++) {
+      const auto& pr = f32ops[i];
+      if ((i < 4 ? c->f32_pinv : c->f32_trans) && !c->cplx) {
         pr.second->alloc(pr.first->n);
         launch_to_float(pr.first->ptr, pr.second->ptr, pr.first->n, c->stream);
       } else {
@@ -703,7 +721,8 @@ int xfmm_set_tree(xfmm_context* c, const xfmm_tree* t) {
     require(t->nsrcs >= 0 && t->ntrgs >= 0, "xfmm_set_tree: negative body count");
     require((t->nsrcs == 0 || t->src_coord) && (t->ntrgs == 0 || t->trg_coord), "xfmm_set_tree: null coordinates");
     std::vector<int> lo(t->level_offset, t->level_offset + D + 2);
-    require(lo[0] == 0 && lo[1] == 1 && lo[D + 1] == nn, "xfmm_set_tree: level_offset must start 0,1 and end at nnodes");
+    // level 0 holds the root, or the roots of a forest of independent trees
+    require(lo[0] == 0 && lo[1] >= 1 && lo[D + 1] == nn, "xfmm_set_tree: level_offset must start 0,>=1 and end at nnodes");
     std::vector<int> lev(nn);
     for (int l = 0; l <= D; l++) {
       require(lo[l + 1] >= lo[l], "xfmm_set_tree: level_offset not monotone");
@@ -749,6 +768,7 @@ int xfmm_set_tree(xfmm_context* c, const xfmm_tree* t) {
     HIPCHK(hipStreamSynchronize(c->stream2));
     c->near_pending = false;
     c->have_tree = false;
+    c->vscale.release();
     c->nnodes = nn;
     c->depth = D;
     c->nsrcs = t->nsrcs;
@@ -1113,6 +1133,7 @@ int xfmm_downward_pass(xfmm_context* c) {
     record(c, 9);
     if (pend) HIPCHK(hipStreamWaitEvent(c->stream, c->ev[20], 0));
     launch_axpy(c->trg_value.ptr, c->near.ptr, c->trg_value.n, c->stream);
+    if (c->vscale.ptr) launch_scale_values(c->trg_value.ptr, c->vscale.ptr, c->ntrgs, c->vw, c->stream);
     c->near_pending = false;
     c->near_overlapped = pend;
     HIPCHK(hipGetLastError());
@@ -1246,6 +1267,134 @@ int xfmm_timings(const xfmm_context* cc, double* ms, int cap) {
   return n;
 }
 
+int xfmm_set_value_scale(xfmm_context* c, const double* scale) {
+  return guarded(c, [&] {
+    require(c->have_tree, "xfmm_set_value_scale: tree not set");
+    drop_graph(c);  // a captured evaluation holds the previous setting
+    if (!scale) {
+      c->vscale.release();
+      return;
+    }
+    for (size_t i = 0; i < size_t(c->ntrgs) * 2; i++)
+      require(std::isfinite(scale[i]), "xfmm_set_value_scale: non-finite factor");
+    c->vscale.upload(scale, size_t(c->ntrgs) * 2, c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int xfmm_set_tuning(xfmm_context* c, int param, int value) {
+  return guarded(c, [&] {
+    if (param == XFMM_TUNE_HAD_KCH) {
+      require(value >= 1 && value <= 256, "xfmm_set_tuning: Hadamard frequency chunk must be in [1, 256]");
+      c->tune_kch = value;
+    } else if (param == XFMM_TUNE_P2P_SPLIT) {
+      require(value >= 0 && value <= 32, "xfmm_set_tuning: P2P split threshold must be in [0, 32]");
+      c->tune_split = value;
+    } else {
+      require(false, "xfmm_set_tuning: unknown parameter");
+    }
+    drop_graph(c);
+  });
+}
+
+int xfmm_get_tuning(const xfmm_context* c, int param, int* value) {
+  if (!c || !value) return fail(nullptr, XFMM_ERR_ARG, "xfmm_get_tuning: null argument");
+  if (param == XFMM_TUNE_HAD_KCH) *value = c->tune_kch;
+  else if (param == XFMM_TUNE_P2P_SPLIT) *value = c->tune_split;
+  else return fail(const_cast<xfmm_context*>(c), XFMM_ERR_ARG, "xfmm_get_tuning: unknown parameter");
+  return XFMM_OK;
+}
+
+int xfmm_autotune(xfmm_context* c, int reps, double* best_ms) {
+  return guarded(c, [&] {
+    check_ready(c);
+    require(reps >= 1 && reps <= 100, "xfmm_autotune: reps must be in [1, 100]");
+    drop_graph(c);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream2));
+    c->near_pending = false;
+    c->down_clean = false;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIPCHK(hipEventCreate(&e0));
+    struct EvGuard {
+      hipEvent_t& a;
+      hipEvent_t& b;
+      ~EvGuard() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+      }
+    } eg{e0, e1};
+    HIPCHK(hipEventCreate(&e1));
+    // fastest of `reps` runs of f (one warm-up run first)
+    auto time_it = [&](auto&& f) {
+      f();
+      float best = 1e30f;
+      for (int r = 0; r < reps; r++) {
+        HIPCHK(hipEventRecord(e0, c->stream));
+        f();
+        HIPCHK(hipEventRecord(e1, c->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float v = 0;
+        HIPCHK(hipEventElapsedTime(&v, e0, e1));
+        best = std::min(best, v);
+      }
+      HIPCHK(hipGetLastError());
+      return double(best);
+    };
+    double ms_split = 0, ms_kch = 0;
+    // near field and L2P of the real kernels: largest tail tile whose
+    // sources are split over lanes (results go to the scratch `near`)
+    if (!c->cplx && c->nt_p2p + c->nt_l2p > 0) {
+      const int cand[] = {0, 8, 16, 32};
+      int best = c->tune_split;
+      double bt = 1e30;
+      for (int v : cand) {
+        c->tune_split = v;
+        const double ms = time_it([&] {
+          DevTree t = c->dev_tree();
+          t.trg_value = c->near.ptr;
+          launch_eval_targets(c->kind, c->f32_p2p, MODE_P2P, t, c->wavek(), c->w_p2p.ptr, c->n_p2p, c->t_p2p.ptr,
+                              c->nt_p2p, c->stream);
+          launch_eval_targets(c->kind, c->f32_far, MODE_L2P, t, c->wavek(), c->w_l2p_eval.ptr, c->n_l2p_eval,
+                              c->t_l2p.ptr, c->nt_l2p, c->stream);
+        });
+        if (ms < bt) {
+          bt = ms;
+          best = v;
+        }
+      }
+      c->tune_split = best;
+      ms_split = bt;
+    }
+    // Hadamard frequency chunk (spectra of the current equivalent densities)
+    if (c->n_m2l_trg > 0) {
+      M2LDev m = c->m2l_dev();
+      DevTree t = c->dev_tree();
+      if (m.amax) launch_zero(c->m2l_amax.ptr, 1, c->stream);
+      launch_fft_up(m, t, c->m2l_src_node.ptr, c->n_m2l_src, c->stream);
+      const int cand[] = {4, 8, 16, 32, 64};
+      int best = c->tune_kch;
+      double bt = 1e30;
+      for (int v : cand) {
+        c->tune_kch = v;
+        const double ms = time_it([&] { run_hadamards(c, m); });
+        if (ms < bt) {
+          bt = ms;
+          best = v;
+        }
+      }
+      c->tune_kch = best;
+      ms_kch = bt;
+    }
+    launch_zero(c->near.ptr, c->near.n, c->stream);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (best_ms) {
+      best_ms[0] = ms_split;
+      best_ms[1] = ms_kch;
+    }
+  });
+}
+
 int xfmm_direct(int kernel, int precision, double wavek_re, double wavek_im, const double* src, const double* q,
                 int ns, const double* trg, int nt, double* trg_value, int gradient) {
   if (ns < 0 || nt < 0) return fail(nullptr, XFMM_ERR_ARG, "xfmm_direct: negative size");
@@ -1280,5 +1429,6 @@ int xfmm_direct(int kernel, int precision, double wavek_re, double wavek_im, con
   }
   return XFMM_OK;
 }
+
 
 }  // extern "C"

@@ -380,6 +380,20 @@ void launch_axpy(double* y, const double* x, size_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_axpy, dim3(unsigned(blocks)), dim3(256), 0, s, y, x, n);
 }
 
+__global__ void k_scale_values(double* __restrict__ val, const double* __restrict__ scale, int n, int vw) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t w = size_t(4) * vw;
+  if (i >= size_t(n) * w) return;
+  const size_t t = i / w, a = (i % w) / vw;
+  val[i] *= scale[2 * t + (a == 0 ? 0 : 1)];
+}
+
+void launch_scale_values(double* val, const double* scale, int n, int vw, hipStream_t s) {
+  const size_t m = size_t(n) * 4 * vw;
+  if (m == 0) return;
+  hipLaunchKernelGGL(k_scale_values, dim3(unsigned((m + 255) / 256)), dim3(256), 0, s, val, scale, n, vw);
+}
+
 __global__ void k_scatter_values(const double* __restrict__ v, const int* __restrict__ perm, int n, int w,
                                  double* __restrict__ out) {
   const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;

@@ -36,6 +36,7 @@ struct DevTree {
   double* up = nullptr;               // T[nnodes][nsurf]
   double* dn = nullptr;               // T[nnodes][nsurf]
   double* trg_value = nullptr;        // T[ntrgs][4]
+  int p2p_split = 32;                 // real-kernel P2P / L2P: tail tiles of <= p2p_split targets (<= 32) split their sources over lanes
 };
 
 enum TargetMode { MODE_P2P = 0, MODE_M2P = 1, MODE_L2P = 2 };
@@ -108,7 +109,7 @@ void launch_fft_up(const M2LDev& m, const DevTree& t, const int* src_node, int n
 // banks).
 constexpr int HAD_UROWS = 18;
 constexpr int HAD_UCAP = 16 * HAD_UROWS;  // union slots per block (LDS staging)
-constexpr int HAD_KCH = 16;    // frequencies per workgroup
+constexpr int HAD_KCH = 16;    // default frequencies per workgroup (tunable per context)
 struct HadBlocks {
   int nblk = 0;
   const int* trg = nullptr;
@@ -119,7 +120,8 @@ struct HadBlocks {
 // m2l_opsx: operators of the level in the expanded MFMA form of launch_m2l_expand
 // hscale != null (mixed, with m.amax set): split-FP16 operators of
 // launch_m2l_expand with their per-frequency scales, spectra scaled by *m.amax
-void launch_hadamard(const M2LDev& m, const void* m2l_opsx, const float* hscale, const HadBlocks& hb,
+// kch: frequencies per workgroup (<= 0: HAD_KCH)
+void launch_hadamard(const M2LDev& m, const void* m2l_opsx, const float* hscale, const HadBlocks& hb, int kch,
                      hipStream_t s);
 // Expand nlev operator levels from the reference layout [L][26][nfreq][64]
 // complex into the per-lane MFMA A-operand form [L][nfreq][26][64 lanes][4]
@@ -138,6 +140,9 @@ size_t m2l_lds_bytes(const M2LDev& m);
 
 void launch_zero(double* p, size_t n, hipStream_t s);
 void launch_axpy(double* y, const double* x, size_t n, hipStream_t s);  // y += x
+// values T[n][4]: value 0 (potential) times scale[i][0], values 1..3
+// (gradient) times scale[i][1]; vw doubles per value
+void launch_scale_values(double* val, const double* scale, int n, int vw, hipStream_t s);
 void launch_scatter_values(const double* sorted_val, const int* perm, int n, int vw, double* out, hipStream_t s);
 
 }  // namespace xfmm

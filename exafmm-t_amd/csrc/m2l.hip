@@ -20,6 +20,7 @@
 // sources overlap in L2) and 16 frequencies; each wave takes 4 frequencies,
 // lanes are targets, the 8x8 matrix M[r][k] is wave-uniform (scalar loads)
 // and the 26 relative positions are accumulated in registers.
+#include <cstdlib>
 #include <type_traits>
 
 #include "kernels.h"
@@ -252,7 +253,7 @@ static void node_dispatch(int p, F&& f) {
 // children g and g+4 of the lane's target source.
 //
 // Workgroup = one block of <= 64 M2L targets (4 waves x 16-target tiles) and
-// HAD_KCH frequencies.  The union of the block's M2L sources (<= HAD_UCAP) is
+// a chunk of kch frequencies (HAD_KCH unless tuned).  The union of the block's M2L sources (<= HAD_UCAP) is
 // staged in LDS one frequency at a time, so each source vector is read from
 // HBM once per block and reused by every target of the block that needs it.
 // Grid order is XCD aware: the workgroups of one frequency chunk run on the
@@ -414,7 +415,7 @@ void launch_m2l_expand(const M2LDev& m, int nlev, const void* M, void* X, float*
                        static_cast<const double*>(M), static_cast<double*>(X));
 }
 
-// Per block of <= 64 target slots and a chunk of HAD_KCH frequencies.  For
+// Per block of <= 64 target slots and a chunk of kch frequencies.  For
 // each frequency the block stages in LDS the spectra of its source union,
 // arranged so that every lane fetches its B operands for one relative
 // position with one 16-byte read: [g][union slot] = {Re c_g, Im c_g,
@@ -429,7 +430,7 @@ void launch_m2l_expand(const M2LDev& m, int nlev, const void* M, void* X, float*
 // for the block and held in registers; an absent source points at a zero
 // slot, so the 26 relative positions run without branches.
 template <typename R>
-__global__ __launch_bounds__(256) void k_hadamard(M2LDev m, const R* __restrict__ opsx, HadBlocks hb, int nchunk) {
+__global__ __launch_bounds__(256) void k_hadamard(M2LDev m, const R* __restrict__ opsx, HadBlocks hb, int nchunk, int kch) {
   typedef typename MfmaT<R>::v4 v4;
   typedef typename Quad<R>::t q4;
   constexpr int US = HAD_UCAP + 1;
@@ -444,7 +445,7 @@ __global__ __launch_bounds__(256) void k_hadamard(M2LDev m, const R* __restrict_
   const int chunk = (i / hb.nblk) * 8 + x;
   const int b = i % hb.nblk;
   if (chunk >= nchunk) return;
-  const int kbeg = chunk * HAD_KCH, kend = min(m.nfreq, kbeg + HAD_KCH);
+  const int kbeg = chunk * kch, kend = min(m.nfreq, kbeg + kch);
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = tid >> 6;
   const int j = lane & 15, g = lane >> 4;
@@ -549,7 +550,8 @@ __global__ __launch_bounds__(256) void k_hadamard(M2LDev m, const R* __restrict_
 // [term][h][union slot] 16-byte vectors.  Blocks, union staging, grid order
 // and result layout are those of k_hadamard.
 __global__ __launch_bounds__(256) void k_hadamard_h(M2LDev m, const h8* __restrict__ X,
-                                                    const float* __restrict__ hscale, HadBlocks hb, int nchunk) {
+                                                    const float* __restrict__ hscale, HadBlocks hb, int nchunk,
+                                                    int kch) {
   typedef float v4 __attribute__((ext_vector_type(4)));
   constexpr int US = HAD_UCAP + 1;
   constexpr int NP = NM2L / 2;
@@ -560,7 +562,7 @@ __global__ __launch_bounds__(256) void k_hadamard_h(M2LDev m, const h8* __restri
   const int chunk = (i / hb.nblk) * 8 + x;
   const int b = i % hb.nblk;
   if (chunk >= nchunk) return;
-  const int kbeg = chunk * HAD_KCH, kend = min(m.nfreq, kbeg + HAD_KCH);
+  const int kbeg = chunk * kch, kend = min(m.nfreq, kbeg + kch);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int j = lane & 15, g = lane >> 4;
   const int nu = hb.nunion[b];
@@ -643,6 +645,125 @@ __global__ __launch_bounds__(256) void k_hadamard_h(M2LDev m, const h8* __restri
     // lane (g, j) register q holds row 4 g + q of target j: (Re, Im) of
     // children 2g and 2g + 1, one 16-byte store
     if (trg >= 0) *reinterpret_cast<v4*>(fout + (size_t(trg) * m.nfreq + k) * NCHILD + 2 * g) = acc;
+  }
+}
+
+// Split-FP16 Hadamard, one wave per workgroup: the wave owns a whole block
+// (<= 64 targets = 4 tiles of 16) and a chunk of kch frequencies.  Same
+// operands, scales, union slots and result layout as k_hadamard_h, but the A
+// operands (operators) come from global memory (L1 / L2; the blocks of one
+// chunk run on one XCD) straight into registers and each serves the 4 target
+// tiles, so LDS carries only the source spectra: written once per frequency
+// and read once per (tile, relative position).  With one wave per
+// workgroup the staging needs no cross-wave barrier; the next frequency's
+// union is loaded into registers while the current one computes.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_hadamard_w(
+    M2LDev m, const h8* __restrict__ X, const float* __restrict__ hscale, HadBlocks hb, int nchunk, int kch) {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  constexpr int US = HAD_UCAP + 1;
+  constexpr int NP = NM2L / 2;
+  constexpr int NT = 4;  // target tiles of 16
+  __shared__ h8 Us[4 * US];  // [term][h][union slot]; slot HAD_UCAP stays zero
+  const int w = blockIdx.x;
+  const int x = w & 7, i = w >> 3;
+  const int chunk = (i / hb.nblk) * 8 + x;
+  const int b = i % hb.nblk;
+  if (chunk >= nchunk) return;
+  const int kbeg = chunk * kch, kend = min(m.nfreq, kbeg + kch);
+  const int lane = threadIdx.x;
+  const int j = lane & 15, g = lane >> 4;
+  const int nu = hb.nunion[b];
+  if (lane < 4) Us[lane * US + HAD_UCAP] = h8{0, 0, 0, 0, 0, 0, 0, 0};
+  // LDS element offsets of the lane's B operands, two 16-bit offsets per word
+  unsigned uo[NT][(NP + 1) / 2];
+#pragma unroll
+  for (int t = 0; t < NT; t++)
+#pragma unroll
+    for (int pm = 0; pm < NP; pm += 2) {
+      unsigned v = 0;
+#pragma unroll
+      for (int h = 0; h < 2 && pm + h < NP; h++) {
+        const int u = hb.loc[(size_t(b) * NM2L + 2 * (pm + h) + (g >> 1)) * 64 + t * 16 + j];
+        v |= unsigned((g & 1) * US + (u >= 0 ? u : HAD_UCAP)) << (16 * h);
+      }
+      uo[t][pm / 2] = v;
+    }
+  const float amax = __uint_as_float(*m.amax);
+  int ex = 0;
+  frexpf(amax, &ex);
+  if (!(amax <= 3.0e38f)) ex = 15;
+  const float sb = ldexpf(1.f, 15 - ex), isb = ldexpf(1.f, ex - 15);
+  const f4* fin = reinterpret_cast<const f4*>(m.fft_in);  // [slot][k][2 halves of 4 children]
+  C2<float>* fout = reinterpret_cast<C2<float>*>(m.fft_out);
+  // staging: element e = lane + 64 q is union slot e / 2, children 4 (e % 2) .. + 3
+  constexpr int NU_Q = (HAD_UCAP * 2 + 63) / 64;
+  const int hh = lane & 1;
+  int usrc[NU_Q];
+#pragma unroll
+  for (int q = 0; q < NU_Q; q++) {
+    const int e = lane + 64 * q;
+    usrc[q] = (e < nu * 2) ? hb.uni[size_t(b) * HAD_UCAP + (e >> 1)] : -1;
+  }
+  f4 ra[NU_Q], rb[NU_Q];
+  auto fetch = [&](int k) {
+#pragma unroll
+    for (int q = 0; q < NU_Q; q++)
+      if (usrc[q] >= 0) {
+        const f4* f = fin + ((size_t(usrc[q]) * m.nfreq + k) * 2 + hh) * 2;
+        ra[q] = f[0];
+        rb[q] = f[1];
+      }
+  };
+  int trg[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) trg[t] = hb.trg[size_t(b) * 64 + t * 16 + j];
+  fetch(kbeg);
+  for (int k = kbeg; k < kend; k++) {
+    // one wave: the barrier orders the previous frequency's LDS reads before
+    // these writes and the writes before the reads below
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NU_Q; q++)
+      if (usrc[q] >= 0) {
+        h8 hi, lo;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          const float v = sb * (e < 4 ? ra[q][e] : rb[q][e - 4]);
+          const _Float16 h = _Float16(v);
+          hi[e] = h;
+          lo[e] = _Float16(v - float(h));
+        }
+        const int slot = (lane + 64 * q) >> 1;
+        Us[hh * US + slot] = hi;
+        Us[2 * US + hh * US + slot] = lo;
+      }
+    __syncthreads();
+    if (k + 1 < kend) fetch(k + 1);
+    const h8* xk = X + size_t(k) * NP * 2 * 64 + lane;
+    v4 ahh[NT], acr[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      ahh[t] = v4{0, 0, 0, 0};
+      acr[t] = v4{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int pm = 0; pm < NP; pm++) {
+      const h8 ah = xk[(2 * pm) * 64], al = xk[(2 * pm + 1) * 64];
+#pragma unroll
+      for (int t = 0; t < NT; t++) {
+        const int off = int((uo[t][pm / 2] >> (16 * (pm & 1))) & 0xffffu);
+        const h8 bh = Us[off], bl = Us[2 * US + off];
+        ahh[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, ahh[t], 0, 0, 0);
+        acr[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acr[t], 0, 0, 0);
+        acr[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acr[t], 0, 0, 0);
+      }
+    }
+    const float sc = hscale[k] * isb;
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+      if (trg[t] >= 0)
+        *reinterpret_cast<v4*>(fout + (size_t(trg[t]) * m.nfreq + k) * NCHILD + 2 * g) = (ahh[t] + acr[t]) * sc;
   }
 }
 
@@ -1053,18 +1174,29 @@ void launch_ifft_dn(const M2LDev& m, const DevTree& t, const int* trg_node, cons
   }
 }
 
-void launch_hadamard(const M2LDev& m, const void* m2l_opsx, const float* hscale, const HadBlocks& hb,
+void launch_hadamard(const M2LDev& m, const void* m2l_opsx, const float* hscale, const HadBlocks& hb, int kch,
                      hipStream_t s) {
   if (hb.nblk <= 0) return;
-  const int nchunk = (m.nfreq + HAD_KCH - 1) / HAD_KCH;
+  if (kch < 1) kch = HAD_KCH;
+  const int nchunk = (m.nfreq + kch - 1) / kch;
   const int nchunk8 = (nchunk + 7) / 8 * 8;
   dim3 grid(unsigned(nchunk8) * unsigned(hb.nblk));
-  if (m.mixed && hscale && m.amax)
-    hipLaunchKernelGGL(k_hadamard_h, grid, dim3(256), 0, s, m, static_cast<const h8*>(m2l_opsx), hscale, hb, nchunk);
+  static const int had_variant = [] {
+    const char* e = std::getenv("XFMM_HAD_VARIANT");
+    return e ? std::atoi(e) : 0;
+  }();
+  if (m.mixed && hscale && m.amax && had_variant == 1)
+    hipLaunchKernelGGL(k_hadamard_w, grid, dim3(64), 0, s, m, static_cast<const h8*>(m2l_opsx), hscale, hb, nchunk,
+                       kch);
+  else if (m.mixed && hscale && m.amax)
+    hipLaunchKernelGGL(k_hadamard_h, grid, dim3(256), 0, s, m, static_cast<const h8*>(m2l_opsx), hscale, hb, nchunk,
+                       kch);
   else if (m.mixed)
-    hipLaunchKernelGGL((k_hadamard<float>), grid, dim3(256), 0, s, m, static_cast<const float*>(m2l_opsx), hb, nchunk);
+    hipLaunchKernelGGL((k_hadamard<float>), grid, dim3(256), 0, s, m, static_cast<const float*>(m2l_opsx), hb, nchunk,
+                       kch);
   else
-    hipLaunchKernelGGL((k_hadamard<double>), grid, dim3(256), 0, s, m, static_cast<const double*>(m2l_opsx), hb, nchunk);
+    hipLaunchKernelGGL((k_hadamard<double>), grid, dim3(256), 0, s, m, static_cast<const double*>(m2l_opsx), hb,
+                       nchunk, kch);
 }
 
 }  // namespace xfmm

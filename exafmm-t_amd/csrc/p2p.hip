@@ -172,7 +172,7 @@ __global__ void k_pack_sources(int n, const double* __restrict__ rel, const doub
   out[i] = v;
 }
 
-// Tail of a leaf with nt <= TILE / 2 targets: k = TILE / nt lanes per target,
+// Tail of a leaf with nt <= t.p2p_split (<= TILE / 2) targets: k = TILE / nt lanes per target,
 // lane (target lane % nt, phase lane / nt) takes every k-th source from its
 // phase on (vector loads); the k partial sums of a target are added in FP64
 // through LDS by split_reduce.
@@ -197,7 +197,8 @@ __device__ __forceinline__ void split_reduce(const R* acc, const R* comp, int nt
 // tile of TILE targets), work[2 b] = leaf, work[2 b + 1] = tile.  A full tile
 // has a target per lane and reads the packed sources (relative to their leaf
 // centre) with scalar loads (constant address space); a tail tile of at most
-// TILE / 2 targets splits the sources of each target over several lanes.
+// t.p2p_split (<= TILE / 2) targets splits the sources of each target over
+// several lanes.
 template <int KIND, typename R>
 __global__ __launch_bounds__(64) void k_p2p_smem(DevTree t, Wavek wk, const int* __restrict__ work) {
   constexpr int NA = 4;
@@ -213,7 +214,7 @@ __global__ __launch_bounds__(64) void k_p2p_smem(DevTree t, Wavek wk, const int*
   const int tb = t.trg_begin[leaf], nt = min(TILE, t.trg_count[leaf] - t0);
   const R kr = R(wk.re), ki = R(wk.im);
   const int g0 = t.p2p_off[leaf], g1 = t.p2p_off[leaf + 1];
-  const bool split = 2 * nt <= TILE;
+  const bool split = nt <= t.p2p_split;
   const int k = split ? TILE / nt : 1;
   const int ti = split ? lane % nt : lane, ph = split ? lane / nt : 0;
   const bool act = split ? ph < k : ti < nt;
@@ -317,7 +318,7 @@ __global__ __launch_bounds__(64) void k_l2p_smem(DevTree t, Wavek wk, const int*
   else tabr = reinterpret_cast<const R*>(t.surf_tab);
   const CRT sp = (CRT)(tabr) + tab0 * 4;
   const CD eq = (CD)(t.dn) + size_t(leaf) * ns;
-  const bool split = 2 * nt <= TILE;
+  const bool split = nt <= t.p2p_split;
   const int k = split ? TILE / nt : 1;
   const int ti = split ? lane % nt : lane, ph = split ? lane / nt : 0;
   const bool act = split ? ph < k : ti < nt;

@@ -70,8 +70,10 @@ Bodies<T> init_targets(py::array_t<real_t, py::array::c_style | py::array::force
 
 // setup (python/exafmm.cpp:155-180): tree + 2:1 balance, lists, M2L_setup,
 // precompute.  The bodies are reordered in place (as in the reference).
+// precompute = false (an addition) skips the operators, for callers that
+// share one operator set between many trees (exafmm.batch).
 template <typename T, typename F>
-Tree<T> setup(Bodies<T>& sources, Bodies<T>& targets, F& fmm) {
+Tree<T> setup(Bodies<T>& sources, Bodies<T>& targets, F& fmm, bool precompute) {
   if (sources.empty() || targets.empty()) throw std::runtime_error("setup: sources and targets must be non-empty");
   Tree<T> tree;
   get_bounds<T>(sources, targets, fmm.x0, fmm.r0);
@@ -81,7 +83,7 @@ Tree<T> setup(Bodies<T>& sources, Bodies<T>& targets, F& fmm) {
   set_colleagues<T>(tree.nodes);
   build_list<T>(tree.nodes, fmm);
   fmm.M2L_setup(tree.nonleafs);
-  fmm.precompute();
+  if (precompute) fmm.precompute();
   return tree;
 }
 
@@ -270,6 +272,15 @@ void bind_fmm(py::module& m, py::class_<F>& c) {
       .def_readonly("depth", &F::depth)
       .def_readonly("r0", &F::r0)
       .def_property_readonly("x0", [](const F& f) { return std::vector<double>{f.x0[0], f.x0[1], f.x0[2]}; })
+      .def(
+          "set_bounds",
+          [](F& f, std::vector<double> x0, double r0) {
+            if (x0.size() != 3 || !(r0 > 0)) throw std::runtime_error("set_bounds: x0 needs 3 values, r0 > 0");
+            for (int d = 0; d < 3; d++) f.x0[d] = x0[d];
+            f.r0 = r0;
+          },
+          "set the root box (centre x0, half side r0) that precompute() uses, without a tree",
+          py::arg("x0"), py::arg("r0"))
       .def_readwrite("filename", &F::filename)
       .def_readwrite("precision", &F::precision,
                      "0: FP64 device path; 1: mixed (FP32 P2P and M2L, FP64 pseudo-inverse solves and "
@@ -284,7 +295,8 @@ void bind_fmm(py::module& m, py::class_<F>& c) {
       .def("sorted_charges", [](F& f, Tree<T>& t) { return sorted_charges<T, F>(f, t); })
       .def("sync_equiv_to_host", [](F& f, Tree<T>& t) { f.sync_equiv_to_host(t.nodes); })
       .def("release_device", &F::release_device);
-  m.def("setup", &setup<T, F>, "setup FMM: tree, 2:1 balance, lists, M2L setup and precomputation");
+  m.def("setup", &setup<T, F>, "setup FMM: tree, 2:1 balance, lists, M2L setup and precomputation",
+        py::arg("sources"), py::arg("targets"), py::arg("fmm"), py::arg("precompute") = true);
   m.def("setup_non_adaptive", &setup_non_adaptive<T, F>,
         "setup FMM on the non-adaptive (uniform) tree of the given depth: tree, lists, M2L setup, precomputation",
         py::arg("sources"), py::arg("targets"), py::arg("fmm"), py::arg("depth"));

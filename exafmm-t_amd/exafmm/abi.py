@@ -12,6 +12,7 @@ import numpy as np
 
 LAPLACE, HELMHOLTZ, MODIFIED_HELMHOLTZ = 0, 1, 2
 FP64, MIXED, FLOAT = 0, 1, 2
+TUNE_HAD_KCH, TUNE_P2P_SPLIT = 0, 1
 LIB_PATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lib",
                         "libexafmm_t_amd.so")
 TIMING_NAMES = ["P2M", "M2M", "P2L", "M2P", "P2P", "M2L", "L2L", "L2P", "FFT", "HAD", "IFFT"]
@@ -68,6 +69,10 @@ def load(path=LIB_PATH):
             "xfmm_values_device": ([vp, ctypes.POINTER(vp)], ctypes.c_int),
             "xfmm_synchronize": ([vp], ctypes.c_int),
             "xfmm_timings": ([vp, vp, ctypes.c_int], ctypes.c_int),
+            "xfmm_set_value_scale": ([vp, vp], ctypes.c_int),
+            "xfmm_set_tuning": ([vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+            "xfmm_get_tuning": ([vp, ctypes.c_int, _c_int_p], ctypes.c_int),
+            "xfmm_autotune": ([vp, ctypes.c_int, vp], ctypes.c_int),
             "xfmm_direct": ([ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, vp, vp, ctypes.c_int,
                              vp, ctypes.c_int, vp, ctypes.c_int], ctypes.c_int),
         }
@@ -245,6 +250,31 @@ class Engine:
         p = ctypes.c_void_p()
         self._check(self.lib.xfmm_values_device(self.h, ctypes.byref(p)), "xfmm_values_device")
         return p.value
+
+    def set_value_scale(self, scale):
+        """Per-target (potential, gradient) factors [ntrgs][2] applied after each evaluation (None removes)."""
+        if scale is None:
+            self._check(self.lib.xfmm_set_value_scale(self.h, None), "xfmm_set_value_scale")
+            return
+        sc = _f64(scale).reshape(-1)
+        if sc.size != 2 * self.ntrgs:
+            raise ValueError("value scale needs 2 factors per target")
+        self._check(self.lib.xfmm_set_value_scale(self.h, _ptr(sc)), "xfmm_set_value_scale")
+
+    def set_tuning(self, param, value):
+        self._check(self.lib.xfmm_set_tuning(self.h, int(param), int(value)), "xfmm_set_tuning")
+
+    def get_tuning(self, param):
+        v = ctypes.c_int()
+        self._check(self.lib.xfmm_get_tuning(self.h, int(param), ctypes.byref(v)), "xfmm_get_tuning")
+        return v.value
+
+    def autotune(self, reps=3):
+        """Runs the engine's autotuner; returns {param: chosen value} and the best times (ms)."""
+        best = np.zeros(2)
+        self._check(self.lib.xfmm_autotune(self.h, int(reps), _ptr(best)), "xfmm_autotune")
+        return ({"had_kch": self.get_tuning(TUNE_HAD_KCH), "p2p_split": self.get_tuning(TUNE_P2P_SPLIT)},
+                {"p2p_l2p_ms": float(best[0]), "hadamard_ms": float(best[1])})
 
     def timings(self):
         ms = np.zeros(len(TIMING_NAMES))

@@ -33,8 +33,6 @@ struct UniformTreeBuilder {
     nd.idx = ni;
     nd.nsrcs = se - sb;
     nd.ntrgs = te - tb;
-    nd.up_equiv.assign(fmm.nsurf, T(0));
-    nd.dn_equiv.assign(fmm.nsurf, T(0));
     nd.key = getKey(get3DIndex(nd.x, nd.level, fmm.x0, fmm.r0), nd.level);
     if (nd.nsrcs == 0 && nd.ntrgs == 0) {  // ghost
       nd.is_leaf = true;
@@ -56,8 +54,8 @@ struct UniformTreeBuilder {
     }
     nd.is_leaf = false;
     const vec3 c = nd.x;
-    std::vector<int> so = split_octants(src.data(), stmp, sb, se, c);
-    std::vector<int> to = split_octants(trg.data(), ttmp, tb, te, c);
+    std::vector<int> so = split_octants(src.data(), stmp.data(), sb, se, c);
+    std::vector<int> to = split_octants(trg.data(), ttmp.data(), tb, te, c);
     if (nodes.size() + NCHILD > nodes.capacity()) throw std::runtime_error("build_uniform_tree: node capacity exceeded");
     const size_t first = nodes.size();
     nodes.resize(first + NCHILD);

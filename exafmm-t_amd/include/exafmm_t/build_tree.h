@@ -10,10 +10,19 @@
 //  * nodes are stored depth-first with the 8 children of a node appended as
 //    one block when the node is split (build_tree.h:147-165),
 //  * 2:1 balance by refining the parents of the colleagues of every refined
-//    node, level by level from the bottom (build_tree.h:247-299).
+//    node, level by level from the bottom (build_tree.h:247-299); the tree is
+//    rebuilt with the balanced leaf keys only when that set differs from the
+//    tree's own (the reference always rebuilds; the result is the same).
+// The octant sorts and the node fill run as OpenMP tasks over subtrees; the
+// node order is fixed by the subtree sizes, not by the schedule.
 #pragma once
-#include <unordered_map>
+#include <array>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
 #include <queue>
+#include <type_traits>
+#include <unordered_map>
 #include "fmm_base.h"
 
 namespace exafmm_t {
@@ -89,79 +98,191 @@ inline int octant_of(const vec3& x, const vec3& c) {
 }
 
 // Stable counting sort of bodies [b,e) by octant of centre c; returns the 9
-// octant boundaries.
+// octant boundaries.  Large ranges count and scatter in parallel (per-thread
+// histograms over consecutive chunks keep the sort stable).
 template <typename T>
-std::vector<int> split_octants(Body<T>* bodies, std::vector<Body<T>>& tmp, int b, int e, const vec3& c) {
-  std::vector<int> cnt(8, 0), off(9, b);
-  for (int i = b; i < e; i++) cnt[octant_of(bodies[i].X, c)]++;
-  for (int o = 0; o < 8; o++) off[o + 1] = off[o] + cnt[o];
-  std::vector<int> pos(off.begin(), off.begin() + 8);
-  for (int i = b; i < e; i++) tmp[pos[octant_of(bodies[i].X, c)]++] = bodies[i];
-  for (int i = b; i < e; i++) bodies[i] = tmp[i];
+std::vector<int> split_octants(Body<T>* bodies, Body<T>* tmp, int b, int e, const vec3& c) {
+  static_assert(std::is_trivially_copyable<Body<T>>::value, "bodies are moved with memcpy");
+  std::vector<int> off(9, b);
+  const int n = e - b;
+  if (n < 200000) {
+    std::vector<int> cnt(8, 0);
+    for (int i = b; i < e; i++) cnt[octant_of(bodies[i].X, c)]++;
+    for (int o = 0; o < 8; o++) off[o + 1] = off[o] + cnt[o];
+    std::vector<int> pos(off.begin(), off.begin() + 8);
+    for (int i = b; i < e; i++) std::memcpy(tmp + pos[octant_of(bodies[i].X, c)]++, bodies + i, sizeof(Body<T>));
+    std::memcpy(bodies + b, tmp + b, sizeof(Body<T>) * size_t(n));
+    return off;
+  }
+  constexpr int kChunks = 64;
+  std::vector<std::array<int, 8>> cnt(kChunks);
+#define XFMM_CHUNK_LO(k) (b + int((long long)n * (k) / kChunks))
+  Body<T>* tp = tmp;
+#pragma omp taskloop grainsize(1) shared(cnt) firstprivate(bodies, b, n, c)
+  for (int k = 0; k < kChunks; k++) {
+    std::array<int, 8> h{};
+    for (int i = XFMM_CHUNK_LO(k); i < XFMM_CHUNK_LO(k + 1); i++) h[octant_of(bodies[i].X, c)]++;
+    cnt[k] = h;
+  }
+  std::array<int, 8> tot{};
+  for (int k = 0; k < kChunks; k++)
+    for (int o = 0; o < 8; o++) tot[o] += cnt[k][o];
+  for (int o = 0; o < 8; o++) off[o + 1] = off[o] + tot[o];
+  // start of chunk k's bodies of octant o
+  std::vector<std::array<int, 8>> start(kChunks);
+  for (int o = 0; o < 8; o++) {
+    int p = off[o];
+    for (int k = 0; k < kChunks; k++) {
+      start[k][o] = p;
+      p += cnt[k][o];
+    }
+  }
+#pragma omp taskloop grainsize(1) shared(start) firstprivate(bodies, tp, b, n, c)
+  for (int k = 0; k < kChunks; k++) {
+    std::array<int, 8> pos = start[k];
+    for (int i = XFMM_CHUNK_LO(k); i < XFMM_CHUNK_LO(k + 1); i++)
+      std::memcpy(tp + pos[octant_of(bodies[i].X, c)]++, bodies + i, sizeof(Body<T>));
+  }
+#pragma omp taskloop grainsize(1) firstprivate(bodies, tp, b, n)
+  for (int k = 0; k < kChunks; k++)
+    std::memcpy(bodies + XFMM_CHUNK_LO(k), tp + XFMM_CHUNK_LO(k),
+                sizeof(Body<T>) * size_t(XFMM_CHUNK_LO(k + 1) - XFMM_CHUNK_LO(k)));
+#undef XFMM_CHUNK_LO
   return off;
 }
+
+// Shape of a subtree, computed in parallel before any Node is written: body
+// ranges after the octant sort and the size of the block of descendants.
+struct Shape {
+  int sb = 0, se = 0, tb = 0, te = 0, level = 0, octant = 0;
+  vec3 x;
+  real_t r = 0;
+  uint64_t key = 0;
+  bool leaf = true;
+  size_t desc = 0;  // descendants (children blocks of the whole subtree)
+  std::unique_ptr<Shape> ch[NCHILD];
+};
+
+// Ranges with more bodies than this are split in tasks of their own.
+constexpr int kTaskBodies = 20000;
 
 template <typename T>
 struct TreeBuilder {
   Bodies<T>& src;
   Bodies<T>& trg;
-  std::vector<Body<T>> stmp, ttmp;
+  // sort scratch: raw storage (bodies are trivially copyable and only ever
+  // memcpy'd in and out)
+  Body<T>* stmp;
+  Body<T>* ttmp;
   Nodes<T>& nodes;
   const Keys& leafkeys;
   FmmBase<T>& fmm;
-  std::vector<std::vector<int>> children;  // per node index: 8 child indices
 
   TreeBuilder(Bodies<T>& s, Bodies<T>& t, Nodes<T>& n, const Keys& lk, FmmBase<T>& f)
-      : src(s), trg(t), stmp(s.size()), ttmp(t.size()), nodes(n), leafkeys(lk), fmm(f) {}
+      : src(s), trg(t), stmp(static_cast<Body<T>*>(std::malloc(sizeof(Body<T>) * (s.size() + 1)))),
+        ttmp(static_cast<Body<T>*>(std::malloc(sizeof(Body<T>) * (t.size() + 1)))), nodes(n), leafkeys(lk), fmm(f) {
+    if (!stmp || !ttmp) throw std::bad_alloc();
+  }
+  ~TreeBuilder() {
+    std::free(stmp);
+    std::free(ttmp);
+  }
+  TreeBuilder(const TreeBuilder&) = delete;
+  TreeBuilder& operator=(const TreeBuilder&) = delete;
 
-  void build(size_t ni, int sb, int se, int tb, int te) {
-    Node<T>& nd = nodes[ni];
-    nd.idx = ni;
-    nd.nsrcs = se - sb;
-    nd.ntrgs = te - tb;
-    nd.up_equiv.assign(fmm.nsurf, T(0));
-    nd.dn_equiv.assign(fmm.nsurf, T(0));
-    nd.key = getKey(get3DIndex(nd.x, nd.level, fmm.x0, fmm.r0), nd.level);
+  // Pass 1: leaf test, octant sort of the node's bodies, children.
+  void shape(Shape* sh) {
+    sh->key = getKey(get3DIndex(sh->x, sh->level, fmm.x0, fmm.r0), sh->level);
     bool leaf_key = true;
-    if (!leafkeys.empty()) {
-      leaf_key = nd.level < int(leafkeys.size()) && leafkeys[nd.level].count(nd.key) > 0;
+    if (!leafkeys.empty())
+      leaf_key = sh->level < int(leafkeys.size()) && leafkeys[sh->level].count(sh->key) > 0;
+    const int ns = sh->se - sh->sb, nt = sh->te - sh->tb;
+    if (ns <= fmm.ncrit && nt <= fmm.ncrit && leaf_key) {
+      sh->leaf = true;
+      sh->desc = 0;
+      return;
     }
-    if (nd.nsrcs <= fmm.ncrit && nd.ntrgs <= fmm.ncrit && leaf_key) {
-      nd.is_leaf = true;
+    sh->leaf = false;
+    std::vector<int> so = split_octants(src.data(), stmp, sh->sb, sh->se, sh->x);
+    std::vector<int> to = split_octants(trg.data(), ttmp, sh->tb, sh->te, sh->x);
+    for (int o = 0; o < NCHILD; o++) {
+      Shape* c = new Shape;
+      sh->ch[o].reset(c);
+      c->r = sh->r / 2;
+      c->x = sh->x;
+      for (int d = 0; d < 3; d++) c->x[d] += c->r * (((o >> d) & 1) * 2 - 1);
+      c->level = sh->level + 1;
+      c->octant = o;
+      c->sb = so[o];
+      c->se = so[o + 1];
+      c->tb = to[o];
+      c->te = to[o + 1];
+    }
+    for (int o = 0; o < NCHILD; o++) {
+      Shape* c = sh->ch[o].get();
+      if ((c->se - c->sb) + (c->te - c->tb) > kTaskBodies) {
+#pragma omp task firstprivate(c)
+        shape(c);
+      } else {
+        shape(c);
+      }
+    }
+#pragma omp taskwait
+    sh->desc = NCHILD;
+    for (int o = 0; o < NCHILD; o++) sh->desc += sh->ch[o]->desc;
+  }
+
+  // Pass 2: node `idx` and its descendants from `base` on: the 8 children of
+  // a split node are one block, followed by the descendant blocks of child
+  // 0, 1, ..., 7 (depth-first storage, build_tree.h:147-165).
+  void fill(const Shape* sh, size_t idx, size_t base, Node<T>* parent) {
+    Node<T>& nd = nodes[idx];
+    nd.idx = idx;
+    nd.x = sh->x;
+    nd.r = sh->r;
+    nd.level = sh->level;
+    nd.octant = sh->octant;
+    nd.parent = parent;
+    nd.key = sh->key;
+    nd.nsrcs = sh->se - sh->sb;
+    nd.ntrgs = sh->te - sh->tb;
+    nd.is_leaf = sh->leaf;
+    if (sh->leaf) {
       nd.trg_value.assign(size_t(nd.ntrgs) * 4, T(0));
-      if (!leafkeys.empty()) {
-        for (int i = sb; i < se; i++) {
-          for (int d = 0; d < 3; d++) nd.src_coord.push_back(src[i].X[d]);
-          nd.isrcs.push_back(src[i].ibody);
-          nd.src_value.push_back(src[i].q);
-        }
-        for (int i = tb; i < te; i++) {
-          for (int d = 0; d < 3; d++) nd.trg_coord.push_back(trg[i].X[d]);
-          nd.itrgs.push_back(trg[i].ibody);
-        }
+      nd.src_coord.resize(size_t(nd.nsrcs) * 3);
+      nd.isrcs.resize(nd.nsrcs);
+      nd.src_value.resize(nd.nsrcs);
+      for (int i = 0; i < nd.nsrcs; i++) {
+        const Body<T>& b = src[sh->sb + i];
+        for (int d = 0; d < 3; d++) nd.src_coord[3 * size_t(i) + d] = b.X[d];
+        nd.isrcs[i] = b.ibody;
+        nd.src_value[i] = b.q;
+      }
+      nd.trg_coord.resize(size_t(nd.ntrgs) * 3);
+      nd.itrgs.resize(nd.ntrgs);
+      for (int i = 0; i < nd.ntrgs; i++) {
+        const Body<T>& b = trg[sh->tb + i];
+        for (int d = 0; d < 3; d++) nd.trg_coord[3 * size_t(i) + d] = b.X[d];
+        nd.itrgs[i] = b.ibody;
       }
       return;
     }
-    nd.is_leaf = false;
-    const vec3 c = nd.x;
-    std::vector<int> so = split_octants(src.data(), stmp, sb, se, c);
-    std::vector<int> to = split_octants(trg.data(), ttmp, tb, te, c);
-    if (nodes.size() + NCHILD > nodes.capacity()) throw std::runtime_error("build_tree: node capacity exceeded");
-    const size_t first = nodes.size();
-    nodes.resize(first + NCHILD);
-    Node<T>& parent = nodes[ni];  // re-fetch: resize does not reallocate (capacity checked)
+    nd.children.resize(NCHILD);
+    for (int o = 0; o < NCHILD; o++) nd.children[o] = &nodes[base + o];
+    size_t next = base + NCHILD;
     for (int o = 0; o < NCHILD; o++) {
-      Node<T>& ch = nodes[first + o];
-      ch.r = parent.r / 2;
-      ch.x = parent.x;
-      for (int d = 0; d < 3; d++) ch.x[d] += ch.r * (((o >> d) & 1) * 2 - 1);
-      ch.parent = &parent;
-      ch.octant = o;
-      ch.level = parent.level + 1;
+      const Shape* c = sh->ch[o].get();
+      const size_t cb = next;
+      next += c->desc;
+      Node<T>* self = &nd;
+      if ((c->se - c->sb) + (c->te - c->tb) > kTaskBodies) {
+#pragma omp task firstprivate(c, cb, self, o)
+        fill(c, base + o, cb, self);
+      } else {
+        fill(c, base + o, cb, self);
+      }
     }
-    parent.children.resize(NCHILD);
-    for (int o = 0; o < NCHILD; o++) parent.children[o] = &nodes[first + o];
-    for (int o = 0; o < NCHILD; o++) build(first + o, so[o], so[o + 1], to[o], to[o + 1]);
+#pragma omp taskwait
   }
 };
 
@@ -173,15 +294,23 @@ struct TreeBuilder {
 template <typename T>
 Nodes<T> build_tree(Bodies<T>& sources, Bodies<T>& targets, NodePtrs<T>& leafs, NodePtrs<T>& nonleafs,
                     FmmBase<T>& fmm, const Keys& leafkeys = Keys()) {
-  Nodes<T> nodes(1);
-  nodes.reserve((sources.size() + targets.size()) * (32 / std::max(fmm.ncrit, 1) + 1) + 64);
-  nodes[0].parent = nullptr;
-  nodes[0].octant = 0;
-  nodes[0].x = fmm.x0;
-  nodes[0].r = fmm.r0;
-  nodes[0].level = 0;
+  detail::Shape root;
+  root.sb = 0;
+  root.se = int(sources.size());
+  root.tb = 0;
+  root.te = int(targets.size());
+  root.x = fmm.x0;
+  root.r = fmm.r0;
+  root.level = 0;
+  Nodes<T> nodes;
   detail::TreeBuilder<T> tb(sources, targets, nodes, leafkeys, fmm);
-  tb.build(0, 0, int(sources.size()), 0, int(targets.size()));
+#pragma omp parallel
+#pragma omp single
+  tb.shape(&root);
+  nodes.resize(1 + root.desc);
+#pragma omp parallel
+#pragma omp single
+  tb.fill(&root, 0, 1, nullptr);
   leafs.clear();
   nonleafs.clear();
   // leafs/nonleafs in creation (depth-first) order, as the reference pushes them
@@ -275,14 +404,19 @@ void balance_tree(Nodes<T>& nodes, Bodies<T>& sources, Bodies<T>& targets, NodeP
                   NodePtrs<T>& nonleafs, FmmBase<T>& fmm) {
   std::unordered_map<uint64_t, size_t> key2id;
   Keys keys = breadth_first_traversal(&nodes[0], key2id);
-  Keys leaf_keys = (nodes.size() == 1) ? keys : find_leaf_keys(balance_tree(keys, key2id, nodes));
+  fmm.depth = int(keys.size()) - 1;
+  if (nodes.size() == 1) return;
+  Keys bkeys = balance_tree(keys, key2id, nodes);
+  // Already balanced (the balanced key set is the tree's own): the rebuild
+  // would reproduce this tree node for node, so it is kept as it is.
+  if (bkeys == keys) return;
+  Keys leaf_keys = find_leaf_keys(bkeys);
   nodes.clear();
   leafs.clear();
   nonleafs.clear();
   nodes = build_tree(sources, targets, leafs, nonleafs, fmm, leaf_keys);
   // leafs/nonleafs hold pointers into the new vector (build_tree returned it by
   // move, so the element storage did not change)
-  fmm.depth = int(keys.size()) - 1;
 }
 #endif  // !EXAFMM_T_NON_ADAPTIVE
 

@@ -82,21 +82,23 @@ typedef struct {
 /* Create a context (FmmBase ctor: p, kernel, wavek; reference
  * include/laplace.h:15, include/helmholtz.h:16, include/modified_helmholtz.h:17).
  * wavek_im is used by the Helmholtz kernel only.  device = HIP device ordinal.
- * precision: XFMM_FP64 computes everything in FP64.  XFMM_MIXED computes the
- * near field (P2P) and the M2L transforms and Hadamard products in FP32 and
- * keeps FP64 wherever a pseudo-inverse amplifies rounding: check potentials
- * (P2M, P2L), the translation GEMMs (P2M/L2P conversions, M2M, L2L) and the
- * evaluation of local equivalent densities at particles (L2P, M2P).  The
- * environment variable XFMM_MIXED_STAGES (comma list of p2p, m2l, fft, far,
- * surf) overrides the FP32 stage set of XFMM_MIXED when a context is created.
- * XFMM_FLOAT is the counterpart of the reference's -DFLOAT build
- * (include/exafmm_t.h:21-37): every evaluation stage computes in FP32 (P2P,
- * M2L, P2M/P2L check potentials, L2P/M2P, the translation GEMMs on FP32
- * MFMA); values stay FP64 in memory.  It expects operators whose
+ * precision: XFMM_FP64 computes everything in FP64.  XFMM_MIXED is the
+ * mixed-precision path: FP32 near field (P2P, Kahan-combined tiles), FP32 M2L
+ * (transforms; Hadamard products on the FP16 matrix cores with split-FP16
+ * operands and FP32 accumulation), FP32 check potentials (P2M, P2L), FP32
+ * L2P / M2P evaluation and FP32 M2M / L2L translations, while the
+ * check-to-equivalent pseudo-inverse solves of P2M and L2P stay FP64 (FP64
+ * operators and storage throughout).  It expects operators whose
  * pseudo-inverses drop singular values below 4e-8 max(S) (the reference's
  * FLOAT EPS, exafmm_t.h:23), which the C++ / Python host layer precomputes
- * for this mode: with untruncated pseudo-inverses FP32 arithmetic in P2M,
- * L2P or the GEMMs loses the result. */
+ * for this mode: those directions only amplify FP32 rounding.  The environment
+ * variable XFMM_MIXED_STAGES (comma list of p2p, m2l, had16, fft, surf, far,
+ * trans, pinv) overrides the FP32 stage set of XFMM_MIXED when a context is
+ * created.  XFMM_FLOAT is the counterpart of the reference's -DFLOAT build
+ * (include/exafmm_t.h:21-37): every evaluation stage computes in FP32,
+ * including the pseudo-inverse solves (FP32 MFMA); values stay FP64 in memory.
+ * It expects the same truncated operators: with untruncated pseudo-inverses
+ * FP32 arithmetic in P2M, L2P or the GEMMs loses the result. */
 int xfmm_create(int kernel, int p, double wavek_re, double wavek_im, int precision, int device,
                 xfmm_context** out);
 void xfmm_destroy(xfmm_context* ctx);
@@ -152,6 +154,36 @@ int xfmm_downward_pass(xfmm_context* ctx);
  * not recorded by replays. */
 int xfmm_graph_capture(xfmm_context* ctx);
 int xfmm_graph_launch(xfmm_context* ctx);
+
+/* Per-target output factors, double[ntrgs][2] in sorted target order: at the
+ * end of every downward pass the potential of target i is multiplied by
+ * scale[2 i] and its gradient by scale[2 i + 1] (NULL removes them).  No
+ * counterpart in the reference; used to evaluate many independent Laplace
+ * problems as one forest of normalised trees (a problem scaled by s into the
+ * shared root box has potential s phi' and gradient s^2 grad'; see the
+ * exafmm.batch module).  Reset by xfmm_set_tree; drops a captured graph. */
+int xfmm_set_value_scale(xfmm_context* ctx, const double* scale);
+
+/* Launch-shape parameters of one context (no counterpart in the reference,
+ * whose loops have fixed blocking; none of them changes which pairs are
+ * summed):
+ *   XFMM_TUNE_HAD_KCH   frequencies per Hadamard workgroup, 1..256 (default 16)
+ *   XFMM_TUNE_P2P_SPLIT real kernels: P2P / L2P tail tiles of at most this many
+ *                       targets (0..32) split each target's sources over
+ *                       several lanes (default 32; the partial sums then add
+ *                       in another order)
+ * Setting one drops a captured graph. */
+enum { XFMM_TUNE_HAD_KCH = 0, XFMM_TUNE_P2P_SPLIT = 1 };
+int xfmm_set_tuning(xfmm_context* ctx, int param, int value);
+int xfmm_get_tuning(const xfmm_context* ctx, int param, int* value);
+
+/* Autotuner: on the uploaded tree and operators, times every candidate of
+ * each tuning parameter (P2P split 0, 8, 16, 32 over the P2P + L2P kernels;
+ * Hadamard chunk 4, 8, 16, 32, 64 over the Hadamard launches), fastest of
+ * `reps` runs each (1..100), and keeps the fastest.  best_ms (may be NULL)
+ * receives the two best times.  Leaves no evaluation result behind (call the
+ * passes again); drops a captured graph, so capture after tuning. */
+int xfmm_autotune(xfmm_context* ctx, int reps, double* best_ms);
 
 /* Individual operators, same order and semantics as the reference
  * (accumulating into the device buffers; no zeroing):

@@ -33,10 +33,8 @@
 #include <stdlib.h>
 #include <string.h>
 
-#define LAPLACE 0
-#define HELMHOLTZ 1
-#define YUKAWA 2
-#define NM2L 26
+#include "oracle.h"
+
 
 static const double INV4PI = 0.0795774715459476678844418816863;
 
@@ -294,37 +292,6 @@ void orc_m2l_operator(int kind, double kre, double kim, int p, double r0, int ch
 
 /* ------------------------------------------------------ evaluation ------ */
 
-typedef struct {
-  int nnodes, depth;
-  double r0;
-  double x0[3];
-  const int* level_offset;
-  const double* center;
-  const int* child0;
-  const int* src_begin;
-  const int* src_count;
-  const int* trg_begin;
-  const int* trg_count;
-  int nsrcs, ntrgs;
-  const double* src_coord;
-  const double* trg_coord;
-  const int* p2p_offset;
-  const int* p2p_source;
-  const int* m2p_offset;
-  const int* m2p_source;
-  const int* p2l_offset;
-  const int* p2l_source;
-  const int* m2l_source;
-} orc_tree;
-
-typedef struct {
-  int kind, p;
-  double kre, kim;
-  int nlevels, nm2l;
-  const double *uc2e_u, *uc2e_v, *dc2e_u, *dc2e_v, *m2m, *l2l, *m2l;
-} orc_ops;
-
-enum { ST_P2M = 1, ST_M2M = 2, ST_P2L = 4, ST_M2P = 8, ST_P2P = 16, ST_M2L = 32, ST_L2L = 64, ST_L2P = 128 };
 
 static int node_level(const orc_tree* t, int n) {
   int l = 0;

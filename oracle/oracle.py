@@ -67,8 +67,11 @@ def load():
                                          ctypes.c_double, ctypes.c_int, ctypes.c_int, _vp]
         lib.orc_evaluate.argtypes = [ctypes.POINTER(OrcTree), ctypes.POINTER(OrcOps), _vp, ctypes.c_int, _vp, _vp,
                                      _vp]
+        lib.orc_fft.argtypes = [ctypes.c_int, ctypes.c_int, _vp, _vp]
+        lib.orc_evaluate_fast.argtypes = [ctypes.POINTER(OrcTree), ctypes.POINTER(OrcOps), _vp, _vp, _vp, _vp, _vp]
+        lib.orc_evaluate_fast.restype = ctypes.c_int
         for f in ("orc_direct", "orc_surface", "orc_surf2conv", "orc_dft3_r2c", "orc_dft3_c2r", "orc_dft3_c2c",
-                  "orc_m2l_operator", "orc_evaluate"):
+                  "orc_m2l_operator", "orc_evaluate", "orc_fft"):
             getattr(lib, f).restype = None
         _lib = lib
     return _lib
@@ -93,6 +96,15 @@ def direct(kernel, src, q, trg, wavek=0.0, gradient=True):
     wk = complex(wavek)
     lib.orc_direct(_kind(kernel), wk.real, wk.imag, _p(src), _p(q.view(np.float64)), src.shape[0], _p(trg),
                    trg.shape[0], _p(out.view(np.float64)), 1 if gradient else 0)
+    return out
+
+
+def fft(x, sign):
+    """Mixed-radix FFT of fast.c: out[k] = sum_j x[j] exp(sign 2 pi i j k / n), n <= 64."""
+    lib = load()
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    out = np.zeros_like(x)
+    lib.orc_fft(x.shape[0], int(sign), _p(x.view(np.float64)), _p(out.view(np.float64)))
     return out
 
 
@@ -204,4 +216,19 @@ class Oracle:
         self.clear()
         self.run(ST_UP)
         self.run(ST_DOWN)
+        return self.val
+
+    FAST_STAGES = ("setup", "P2M", "M2M", "P2L+M2P", "P2P", "FFT", "HAD", "IFFT", "L2L", "L2P")
+
+    def evaluate_fast(self):
+        """The FFT-based CPU evaluation of fast.c (Laplace, one operator level): same result as evaluate().
+        Seconds per stage in self.fast_stage_s."""
+        if self.kernel != LAPLACE:
+            raise ValueError("evaluate_fast: Laplace only")
+        st = np.zeros(len(self.FAST_STAGES))
+        rc = self.lib.orc_evaluate_fast(ctypes.byref(self.t), ctypes.byref(self.o), _p(self.q), _p(self.up),
+                                        _p(self.dn), _p(self.val), _p(st))
+        if rc != 0:
+            raise ValueError("evaluate_fast: unsupported operators")
+        self.fast_stage_s = dict(zip(self.FAST_STAGES, st.tolist()))
         return self.val

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from fmm_cases import kernel_np, make_problem, rel_err
+from fmm_cases import equiv_rel_diff, kernel_np, make_problem, rel_err
 
 KERNELS = [(O.LAPLACE, 0.0), (O.YUKAWA, 3.0), (O.HELMHOLTZ, 4.0 + 1.5j)]
 
@@ -205,3 +205,37 @@ def _fmm_vs_direct(pr, p, tol):
     d = O.direct(pr["kid"], pr["flat"]["src_coord"].reshape(-1, 3), pr["qs"], tc[idx], pr["wavek"])
     assert rel_err(v[idx, 0], d[:, 0]) < tol
     assert rel_err(v[idx, 1:], d[:, 1:]) < 5 * tol
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 8, 10, 12, 16, 20, 24, 30])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_mixed_radix_fft_matches_numpy(n, sign):
+    """The CPU baseline's own FFT (oracle/fast.c) against numpy.fft (radix 2, 3, 4, 5 and their mixes)."""
+    x = np.random.default_rng(n).normal(size=n) + 1j * np.random.default_rng(n + 1).normal(size=n)
+    ref = np.fft.fft(x) if sign < 0 else np.fft.ifft(x) * n
+    assert rel_err(O.fft(x, sign), ref) < 1e-14
+
+
+@pytest.mark.parametrize("p,dist,n", [(4, "cube", 4000), (5, "plummer", 3000), (6, "clusters", 5000),
+                                      (8, "cube", 6000)])
+def test_fast_cpu_evaluation_equals_oracle(p, dist, n):
+    """The FFT-based CPU evaluation (bench.py's CPU baseline) against the plain
+    restatement on the same tree and operators, adaptive trees with P2L / M2P
+    lists included (odd p exercises the GEMM's column tail)."""
+    pr = make_problem("laplace", n, p, ncrit=32, dist=dist, seed=4, charge="u11")
+    orc = O.Oracle(pr["kid"], p, pr["flat"], pr["ops"])
+    orc.set_charges(pr["qs"])
+    ref = orc.evaluate().copy()
+    up, dn = orc.up.copy(), orc.dn.copy()
+    v = orc.evaluate_fast()
+    assert rel_err(v, ref) < 1e-12
+    # equivalent densities come out of the ill-conditioned pseudo-inverses,
+    # which amplify last-bit differences of the check potentials (other
+    # summation order in the FFT and the GEMMs) into near-null directions:
+    # compared by the check potentials they generate
+    f = pr["flat"]
+    lev = np.repeat(np.arange(len(f["level_offset"]) - 1), np.diff(f["level_offset"]))
+    rows = np.nonzero((f["src_count"] > 0) | (f["child0"] >= 0))[0]
+    assert equiv_rel_diff(0, 0.0, p, f["r0"], lev, orc.up, up, rows, True) < 1e-12
+    leaf = np.nonzero((f["child0"] < 0) & (f["trg_count"] > 0))[0]
+    assert equiv_rel_diff(0, 0.0, p, f["r0"], lev, orc.dn, dn, leaf, False) < 1e-12
