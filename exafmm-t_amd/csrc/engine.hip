@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -463,6 +464,32 @@ void op_M2L(xfmm_context* c, bool timed) {
 }  // namespace
 
 // =================================================================== C ABI
+
+// ---------------------------------------------------------- allocation-free direct sum
+//
+// Partial sums of xfmm_direct_noalloc live in this file-scope array, one copy
+// per device, for the life of the process.  Host code never passes the name
+// `scratch` to a kernel (in host code it is a host-side stand-in); it uses the
+// device address that hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) double scratch[XFMM_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up once per device).
+double* scratch_ptr(int dev) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw HipError{"hipGetSymbolAddress(scratch) returned a null address"};
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
 
 extern "C" {
 
@@ -1430,5 +1457,43 @@ int xfmm_direct(int kernel, int precision, double wavek_re, double wavek_im, con
   return XFMM_OK;
 }
 
+
+int xfmm_direct_noalloc(int kernel, int precision, double wavek_re, double wavek_im, const double* src,
+                        const double* q, int ns, const double* trg, int nt, double* trg_value, int gradient,
+                        void* stream) {
+  if (ns < 0 || nt < 0) return fail(nullptr, XFMM_ERR_ARG, "xfmm_direct_noalloc: negative size");
+  if (kernel < XFMM_LAPLACE || kernel > XFMM_MODIFIED_HELMHOLTZ)
+    return fail(nullptr, XFMM_ERR_ARG, "xfmm_direct_noalloc: unknown kernel");
+  if (nt == 0) return XFMM_OK;
+  if ((ns && (!src || !q)) || !trg || !trg_value) return fail(nullptr, XFMM_ERR_ARG, "xfmm_direct_noalloc: null pointer");
+  const int vw = kernel == XFMM_HELMHOLTZ ? 2 : 1;
+  const int na = (gradient ? 4 : 1) * vw;
+  constexpr size_t cap = XFMM_NOALLOC_SCRATCH_BYTES / sizeof(double);
+  const size_t per_split = size_t(nt) * na;
+  if (per_split > cap)
+    return fail(nullptr, XFMM_ERR_ARG,
+                "xfmm_direct_noalloc: " + std::to_string(nt) + " targets x " + std::to_string(na) +
+                    " values exceed the " + std::to_string(XFMM_NOALLOC_SCRATCH_BYTES) + "-byte scratch buffer (at most " +
+                    std::to_string(cap / na) + " targets for this kernel and output)");
+  try {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(nullptr, XFMM_ERR_ARG, "xfmm_direct_noalloc: device ordinal out of range");
+    double* part = scratch_ptr(dev);
+    // same source split as xfmm_direct, capped by the scratch capacity
+    const int tblocks = (nt + 63) / 64;
+    int nsplit = std::max(1, std::min(2048 / tblocks + 1, (ns + 255) / 256));
+    nsplit = int(std::min<size_t>(size_t(nsplit), cap / per_split));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    launch_direct(kernel, precision != XFMM_FP64, vw == 2, Wavek{wavek_re, wavek_im}, src, q, ns, trg, nt, trg_value,
+                  gradient, part, nsplit, s);
+    HIPCHK(hipGetLastError());
+  } catch (const HipError& e) {
+    return fail(nullptr, XFMM_ERR_HIP, e.msg);
+  } catch (...) {
+    return fail(nullptr, XFMM_ERR_HIP, "xfmm_direct_noalloc: unexpected error");
+  }
+  return XFMM_OK;
+}
 
 }  // extern "C"

@@ -226,6 +226,27 @@ int xfmm_timings(const xfmm_context* ctx, double* ms, int cap);
 int xfmm_direct(int kernel, int precision, double wavek_re, double wavek_im, const double* src,
                 const double* q, int ns, const double* trg, int nt, double* trg_value, int gradient);
 
+/* Size of the device scratch array used by xfmm_direct_noalloc (32 MiB per
+ * device, held from the first call on that device until the process ends). */
+#define XFMM_NOALLOC_SCRATCH_BYTES (32u << 20)
+
+/* Allocation-free variant of xfmm_direct.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels, arguments and result (+= into trg_value) as xfmm_direct, except
+ * that src, q, trg and trg_value are DEVICE pointers on the current device
+ * and the work is only enqueued on `stream` (a hipStream_t; NULL = the null
+ * stream): the call returns before the result is ready.  It calls no
+ * hipMalloc / hipFree; the per-source-split partial sums go to a file-scope
+ * device array of XFMM_NOALLOC_SCRATCH_BYTES, so nt * (gradient ? 4 : 1) *
+ * (Helmholtz ? 2 : 1) doubles must fit in it (at most 1,048,576 targets for a
+ * real kernel with gradient), else XFMM_ERR_ARG.  Every call on one device
+ * shares that device's array: calls that overlap on one device (other
+ * streams, other threads) race and must be serialised by the caller. */
+int xfmm_direct_noalloc(int kernel, int precision, double wavek_re, double wavek_im, const double* src,
+                        const double* q, int ns, const double* trg, int nt, double* trg_value, int gradient,
+                        void* stream);
+
 #ifdef __cplusplus
 }
 #endif
