@@ -4,13 +4,14 @@ One step = upward_pass + downward_pass of the whole tree with charges already
 resident in HBM (the FMM evaluation phase, every operator including the
 near-field P2P, which each upward_pass starts anew on a second stream; the
 host tree/list/precompute setup is reported separately).  The headline `value` is the mixed-precision
-path of this workload: FP32 near-field P2P (Kahan-combined 64-source tiles) and FP32 M2L (transforms, and
-the Hadamard products on the f16 matrix cores with split-FP16 operands and FP32 accumulation), with the
-check-to-equivalent pseudo-inverse solves, the M2M / L2L translations, the operators and all storage in
-FP64; its pseudo-inverses drop the singular values below 4e-8 max(S).  Timed beside it: the FLOAT mode (the
-counterpart of the reference's -DFLOAT build: every evaluation stage in FP32, translation GEMMs on FP32
-MFMA) and the plain FP64 path (untruncated operators).  Errors are relative L2 against an FP64 direct sum
-on 2,000 seeded sample targets computed by the engine's direct-sum kernel.
+path of this workload: FP32 near-field P2P (Kahan-combined 64-source tiles), FP32 M2L (transforms, and the
+Hadamard products on the f16 matrix cores with split-FP16 operands and FP32 accumulation), FP32 check
+potentials, L2P / M2P evaluation and M2M / L2L translations, with the check-to-equivalent pseudo-inverse
+solves of P2M and L2P, the operators and all storage in FP64; its pseudo-inverses drop the singular values
+below 4e-8 max(S).  Timed beside it: the FLOAT mode (the counterpart of the reference's -DFLOAT build: the
+solves in FP32 too) and the plain FP64 path (untruncated operators).  Errors are relative L2 against an FP64
+direct sum on 2,000 seeded sample targets computed by the engine's direct-sum kernel.  The CPU baseline is
+oracle/fast.c (the FFT-based CPU evaluation) on the same tree and operators, timed on the host cores.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--n 2000000] [--no-cpu-baseline]
 
@@ -18,7 +19,8 @@ The other workloads of the same evaluation are measured with --workload (one JSO
 headline): laplace_small (N=100k, p=8, ncrit=64, charges U[0,1)), yukawa_sphere (k=5, N=1M on the
 unit sphere, charges N(0,1), p=8, ncrit=64), helmholtz_cube (k=20, N=4M uniform, complex charges,
 p=10, ncrit=128) and laplace_batched (64 Plummer problems, N log-uniform in [1e4, 5e5], charges 1/N,
-p=8, ncrit=64; every problem's evaluation captured in a HIP graph, all graphs replayed back to back).
+p=8, ncrit=64; all problems merged into one forest, one launch per operator, the launch shapes picked by
+the autotuner, one captured graph replayed per step).
 """
 import argparse
 import json
@@ -41,13 +43,14 @@ PEAK = {"fp32_vector": 157.3e12, "fp64_vector": 78.6e12, "fp32_mfma": 157.3e12, 
         "fp16_mfma": 2.5e15, "hbm": 8.0e12}
 
 
-DTYPE = {"mixed": "f32 P2P (Kahan across tiles) and M2L (f32 transforms; Hadamard on f16 matrix cores, split-f16 "
-                  "operands, f32 accumulation) + f64 pseudo-inverse solves (truncated at 4e-8 max(S)), f64 "
-                  "M2M/L2L, operators and storage",
-         "float": "f32 arithmetic in every evaluation stage (f16-split Hadamard, f32 MFMA translation GEMMs), "
-                  "f64 storage and setup; pseudo-inverses truncated at 4e-8 max(S)",
+DTYPE = {"mixed": "f32 P2P (Kahan across tiles), f32 M2L (transforms; Hadamard on f16 matrix cores with split-f16 "
+                  "operands and f32 accumulation), f32 check potentials, L2P evaluation and M2M / L2L; f64 "
+                  "pseudo-inverse solves of P2M / L2P (pseudo-inverses truncated at 4e-8 max(S)); f64 operators "
+                  "and storage",
+         "float": "f32 arithmetic in every evaluation stage incl. the pseudo-inverse solves (f16-split Hadamard, "
+                  "f32 MFMA GEMMs), f64 storage and setup; pseudo-inverses truncated at 4e-8 max(S)",
          "fp64": "f64"}
-PRECISION_LABEL = {"mixed": "mixed: f32 P2P/M2L + f64 pseudo-inverse solves",
+PRECISION_LABEL = {"mixed": "mixed: f32 P2P, M2L, check potentials, L2P, M2M/L2L + f64 pseudo-inverse solves",
                    "float": "float: every evaluation stage f32", "fp64": "fp64"}
 PATHS = (("mixed", 1), ("float", 2), ("fp64", 0))  # abi.MIXED, abi.FLOAT, abi.FP64; the first is the headline
 
