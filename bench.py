@@ -90,15 +90,18 @@ def setup_problem(n, p, ncrit, seed):
     q = rng.uniform(-1.0, 1.0, n)
     fmm = L.LaplaceFmm(p, ncrit, os.path.join(tempfile.gettempdir(), "bench_laplace_p%d.dat" % p))
     t0 = time.perf_counter()
-    tree = L.setup(L.init_sources(x, q), L.init_targets(x), fmm)
+    tree = L.setup(L.init_sources(x, q), L.init_targets(x), fmm, precompute=False)
     t_setup = time.perf_counter() - t0
     t0 = time.perf_counter()
     flat = fmm.flat_tree(tree)
-    ops = fmm.operators()
     qs = fmm.sorted_charges(tree)
     t_flat = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    fmm.precompute()
+    ops = fmm.operators()
+    t_pre = time.perf_counter() - t0
     return dict(x=x, q=q, flat=flat, ops=ops, ops_trunc=truncated_operators(fmm), qs=qs, depth=fmm.depth,
-                t_setup=t_setup, t_flat=t_flat)
+                t_setup=t_setup, t_flat=t_flat, t_pre=t_pre)
 
 
 def truncated_operators(fmm):
@@ -444,6 +447,62 @@ def roofline(dom, ms, mixed, h16, npairs, m2l_pairs, nfreq, nsrc, ntrg):
     return roof
 
 
+def pinv_rank(ops, key, nsurf):
+    """Rank of a truncated pseudo-inverse factor: 1 + the last column of C2E_V (nsurf x nsurf per level,
+    row-major) that is not exactly zero, the number of singular directions the engine applies."""
+    v = np.asarray(ops[key]).reshape(-1, nsurf, nsurf)
+    nz = np.nonzero(np.any(v != 0.0, axis=(0, 1)))[0]
+    return int(nz[-1]) + 1 if nz.size else 1
+
+
+def evaluation_roofline(flat, ops, p, precision, kernels_ms, ms, npairs, m2l_pairs, nfreq, nsrc, ntrg):
+    """Roofline of the whole evaluation: for every stage its own roofline time (algorithmic flops / the
+    peak of the unit it runs on, or algorithmic bytes / HBM peak, whichever is longer), their sum (a
+    serial lower bound: every stage fills the chip on its own), and that sum over the measured time per
+    step.  Precision per stage follows the path: FP64 everywhere, or the mixed policy (FP32 near field,
+    check potentials, L2P evaluation, M2M / L2L, transforms; split-FP16 Hadamard; FP64 pseudo-inverse
+    solves), or FLOAT (every stage FP32)."""
+    nsurf = 6 * (p - 1) ** 2 + 2
+    leaf = flat["child0"] < 0
+    nleaf_s = int(np.sum(leaf & (flat["src_count"] > 0)))
+    nleaf_t = int(np.sum(leaf & (flat["trg_count"] > 0)))
+    nonleaf = int(np.sum(~leaf))
+    ns, nt = int(np.sum(flat["src_count"])), int(np.sum(flat["trg_count"]))
+    r_up, r_dn = pinv_rank(ops, "uc2e_v", nsurf), pinv_rank(ops, "dc2e_v", nsurf)
+    f32 = precision != 0
+    valu = PEAK["fp32_vector"] if f32 else PEAK["fp64_vector"]
+    trans = PEAK["fp32_mfma"] if f32 else PEAK["fp64_mfma"]
+    pinv = PEAK["fp32_mfma"] if precision == 2 else PEAK["fp64_mfma"]
+    rb = 4 if f32 else 8
+    had_peak = PEAK["fp16_mfma"] / 3 if f32 else PEAK["fp64_mfma"]
+    # stage -> kernels -> views (unit, seconds); a kernel takes its longest view, a stage the sum of its kernels
+    stages = {
+        # check potentials (12 flop per pair), then the two pseudo-inverse GEMMs over r directions
+        "P2M": [[("valu", 12.0 * ns * nsurf / valu)], [("mfma", 4.0 * nleaf_s * nsurf * r_up / pinv)]],
+        "M2M": [[("mfma", 16.0 * nonleaf * nsurf * nsurf / trans)]],
+        "P2P": [[("valu", float(P2P_FLOP_PER_PAIR) * npairs / valu)]],
+        "FFT": [[("hbm", (nsrc * nfreq * 8 * 2 * rb + nsrc * 8 * nsurf * 8) / PEAK["hbm"])]],
+        "HAD": [[("hbm", ((nsrc + ntrg) * nfreq * 8 * 2 * rb + 26 * nfreq * 128 * rb) / PEAK["hbm"]),
+                 ("mfma", float(HAD_FLOP_PER_PAIR_FREQ) * m2l_pairs * nfreq / had_peak)]],
+        "IFFT": [[("hbm", (ntrg * nfreq * 8 * 2 * rb + 2 * ntrg * 8 * nsurf * 8) / PEAK["hbm"])]],
+        "L2L": [[("mfma", 16.0 * nonleaf * nsurf * nsurf / trans)]],
+        "L2P": [[("mfma", 4.0 * nleaf_t * nsurf * r_dn / pinv)],
+                [("valu", float(P2P_FLOP_PER_PAIR) * nt * nsurf / valu)]],
+    }
+    out, total = {}, 0.0
+    for name, kernels in stages.items():
+        best = [max(views, key=lambda q: q[1]) for views in kernels]
+        t = sum(x for _, x in best)
+        out[name] = {"bound": "+".join(b for b, _ in best), "roofline_ms": round(1e3 * t, 4),
+                     "ms": kernels_ms.get(name)}
+        total += t
+    return {"serial_roofline_ms": round(1e3 * total, 4), "ms": ms, "frac": 1e3 * total / ms,
+            "pinv_rank": [r_up, r_dn], "stages": out,
+            "note": "per stage: algorithmic flops / peak of its unit or bytes / 8 TB/s; FP32 vector peak 157.3, "
+                    "FP64 78.6, FP32 MFMA 157.3, FP64 MFMA 78.6, split-FP16 Hadamard 2500/3 TFLOP/s; the FFT "
+                    "stage's ms covers the near field running beside it on the second stream"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -478,8 +537,8 @@ def main():
     log("setup: N=%d p=%d ncrit=%d" % (a.n, a.p, a.ncrit))
     prob = setup_problem(a.n, a.p, a.ncrit, seed=0)
     flat = prob["flat"]
-    log("setup %.2fs flatten %.2fs depth %d nodes %d" % (prob["t_setup"], prob["t_flat"], prob["depth"],
-                                                          flat["nnodes"]))
+    log("setup %.2fs flatten %.2fs precompute %.2fs depth %d nodes %d"
+        % (prob["t_setup"], prob["t_flat"], prob["t_pre"], prob["depth"], flat["nnodes"]))
     npairs = p2p_pairs(flat)
     m2l_pairs = int(np.sum(flat["m2l_source"] >= 0))
     n1 = 2 * a.p
@@ -538,8 +597,10 @@ def main():
         "err_phi": head["err"][0],
         "err_grad": head["err"][1],
         "kernels_ms": kt,
-        "setup_s": round(prob["t_setup"] + prob["t_flat"], 3),
+        "setup_s": round(prob["t_setup"] + prob["t_flat"], 3), "precompute_s": round(prob["t_pre"], 3),
         "roofline": roof,
+        "evaluation_roofline": evaluation_roofline(flat, operators_for(prob, paths[0][1]), a.p, paths[0][1], kt,
+                                                   ms_step, npairs, m2l_pairs, nfreq, nsrc, ntrg),
     }
     line.update(compare_fields(res, hname))
     if not a.no_cpu_baseline:

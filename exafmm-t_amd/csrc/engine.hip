@@ -105,7 +105,7 @@ struct xfmm_context {
   // numerical rank of the upward / downward pseudo-inverse per operator level
   // (uc2e_uT / dc2e_uT hold only the first r_up / r_dn columns, row stride r)
   std::vector<int> r_up, r_dn;
-  DevBuf<float> uc2e_uT32, uc2e_vT32, dc2e_uT32, dc2e_vT32, m2mcat32, l2lcat32;  // f32_pinv / f32_trans (real kernels)
+  DevBuf<float> uc2e_uT32, uc2e_vT32, dc2e_uT32, dc2e_vT32, m2mcat32, l2lcat32;  // f32_pinv / f32_trans
   DevBuf<double> m2l64;  // M2L operators in the expanded MFMA form, [L][nfreq][26][64][4]
   DevBuf<float> m2l32;   // FP32 form, or the split-FP16 form of the same size when f16_had
   DevBuf<float> m2l_hscale;  // f16_had: operator scale per [L][nfreq]
@@ -311,19 +311,27 @@ void check_ready(xfmm_context* c) {
   if (c->nm2l != 1 && c->nm2l < c->depth) throw ArgError{"xfmm: M2L operators cover fewer levels than the tree depth"};
 }
 
-// B operand of a translation GEMM: the FP32 copy for FP32 GEMMs of the real
-// kernels, else the FP64 operator; off = element offset.
+// B operand of a translation GEMM: the FP32 copy for FP32 GEMMs, else the
+// FP64 operator; off = element offset.
 const void* gemm_op(bool f32, const xfmm_context* c, const DevBuf<double>& d, const DevBuf<float>& f, size_t off) {
-  if (f32 && !c->cplx) return f.ptr + off;
+  (void)c;
+  if (f32) return f.ptr + off;
   return d.ptr + off;
 }
+
+// Doubles per nsurf x nsurf operator in GEMM form: complex operators are held
+// as real 2 x 2 blocks [[Re, Im], [-Im, Re]] per entry, so that a complex
+// row-GEMM on interleaved (re, im) rows is one real GEMM with K and N doubled
+// (on the FP64 matrix cores like the real kernels).
+size_t gemm_op_size(const xfmm_context* c) { return size_t(c->nsurf) * c->nsurf * c->vw * c->vw; }
 
 void op_P2M(xfmm_context* c) {
   // check potentials of every leaf, then up = alpha * UC2E_V (UC2E_U check)
   DevTree t = c->dev_tree();
   launch_eval_surface(c->kind, c->f32_surf, MODE_P2M, t, c->wavek(), c->w_leaf.ptr, c->n_leaf, c->check.ptr,
                       c->stream);
-  const size_t ns2 = size_t(c->nsurf) * c->nsurf * c->vw;
+  const size_t ns2 = gemm_op_size(c);
+  const int vw = c->vw;
   for (int l = 0; l <= c->depth; l++) {
     const int b = c->leaf_level_off[l], e = c->leaf_level_off[l + 1];
     if (e <= b) continue;
@@ -331,52 +339,54 @@ void op_P2M(xfmm_context* c) {
     // only the first r singular directions carry a nonzero S^+ entry: the
     // first product computes those r coefficients, the second sums over them
     const int r = c->r_up[L];
-    launch_gemm_rows(c->cplx, c->f32_pinv, e - b, r, c->nsurf, c->check.ptr, c->leaf_slot_rows.ptr + b,
+    launch_gemm_rows(c->f32_pinv, e - b, r * vw, c->nsurf * vw, c->check.ptr, c->leaf_slot_rows.ptr + b,
                      gemm_op(c->f32_pinv, c, c->uc2e_uT, c->uc2e_uT32, L * ns2), c->tmp.ptr, c->leaf_slot_rows.ptr + b,
                      1.0, 0, c->stream);
-    launch_gemm_rows(c->cplx, c->f32_pinv, e - b, c->nsurf, r, c->tmp.ptr, c->leaf_slot_rows.ptr + b,
+    launch_gemm_rows(c->f32_pinv, e - b, c->nsurf * vw, r * vw, c->tmp.ptr, c->leaf_slot_rows.ptr + b,
                      gemm_op(c->f32_pinv, c, c->uc2e_vT, c->uc2e_vT32, L * ns2), c->up.ptr, c->leaf_node_rows.ptr + b,
                      c->leaf_scale(l), 0, c->stream);
   }
 }
 
 void op_M2M(xfmm_context* c) {
-  const size_t blk = size_t(8) * c->nsurf * c->nsurf * c->vw;
+  const size_t blk = 8 * gemm_op_size(c);
+  const int vw = c->vw;
   for (int l = c->depth - 1; l >= 0; l--) {
     const int b = c->nl_level_off[l], e = c->nl_level_off[l + 1];
     if (e <= b) continue;
     const int L = c->op_level(l);
     // parent row = [child_0 .. child_7] (contiguous, K = 8 nsurf) x M2Mcat
-    launch_gemm_rows(c->cplx, c->f32_trans, e - b, c->nsurf, 8 * c->nsurf, c->up.ptr, c->nl_child_rows.ptr + b,
-                     gemm_op(c->f32_trans, c, c->m2mcat, c->m2mcat32, L * blk), c->up.ptr, c->nl_rows.ptr + b, 1.0, 1, c->stream,
-                     c->gemm_scratch.ptr, c->gemm_scratch.n);
+    launch_gemm_rows(c->f32_trans, e - b, c->nsurf * vw, 8 * c->nsurf * vw, c->up.ptr, c->nl_child_rows.ptr + b,
+                     gemm_op(c->f32_trans, c, c->m2mcat, c->m2mcat32, L * blk), c->up.ptr, c->nl_rows.ptr + b, 1.0, 1,
+                     c->stream, c->gemm_scratch.ptr, c->gemm_scratch.n);
   }
 }
 
 void op_L2L(xfmm_context* c) {
-  const size_t blk = size_t(8) * c->nsurf * c->nsurf * c->vw;
+  const size_t blk = 8 * gemm_op_size(c);
+  const int vw = c->vw;
   for (int l = 0; l < c->depth; l++) {
     const int b = c->nl_level_off[l], e = c->nl_level_off[l + 1];
     if (e <= b) continue;
     const int L = c->op_level(l);
-    launch_gemm_rows(c->cplx, c->f32_trans, e - b, 8 * c->nsurf, c->nsurf, c->dn.ptr, c->nl_rows.ptr + b,
-                     gemm_op(c->f32_trans, c, c->l2lcat, c->l2lcat32, L * blk),
-                     c->dn.ptr, c->nl_child_rows.ptr + b, 1.0, 1, c->stream, c->gemm_scratch.ptr,
-                     c->gemm_scratch.n);
+    launch_gemm_rows(c->f32_trans, e - b, 8 * c->nsurf * vw, c->nsurf * vw, c->dn.ptr, c->nl_rows.ptr + b,
+                     gemm_op(c->f32_trans, c, c->l2lcat, c->l2lcat32, L * blk), c->dn.ptr, c->nl_child_rows.ptr + b,
+                     1.0, 1, c->stream, c->gemm_scratch.ptr, c->gemm_scratch.n);
   }
 }
 
 void op_L2P(xfmm_context* c) {
-  const size_t ns2 = size_t(c->nsurf) * c->nsurf * c->vw;
+  const size_t ns2 = gemm_op_size(c);
+  const int vw = c->vw;
   for (int l = 0; l <= c->depth; l++) {
     const int b = c->leaf_level_off[l], e = c->leaf_level_off[l + 1];
     if (e <= b) continue;
     const int L = c->op_level(l);
     const int r = c->r_dn[L];
-    launch_gemm_rows(c->cplx, c->f32_pinv, e - b, r, c->nsurf, c->dn.ptr, c->leaf_node_rows.ptr + b,
+    launch_gemm_rows(c->f32_pinv, e - b, r * vw, c->nsurf * vw, c->dn.ptr, c->leaf_node_rows.ptr + b,
                      gemm_op(c->f32_pinv, c, c->dc2e_uT, c->dc2e_uT32, L * ns2), c->tmp.ptr, c->leaf_slot_rows.ptr + b,
                      1.0, 0, c->stream);
-    launch_gemm_rows(c->cplx, c->f32_pinv, e - b, c->nsurf, r, c->tmp.ptr, c->leaf_slot_rows.ptr + b,
+    launch_gemm_rows(c->f32_pinv, e - b, c->nsurf * vw, r * vw, c->tmp.ptr, c->leaf_slot_rows.ptr + b,
                      gemm_op(c->f32_pinv, c, c->dc2e_vT, c->dc2e_vT32, L * ns2), c->dn.ptr, c->leaf_node_rows.ptr + b,
                      c->leaf_scale(l), 0, c->stream);
   }
@@ -666,17 +676,46 @@ int xfmm_set_operators(xfmm_context* c, int nlevels, const double* uc2e_u, const
             for (int w = 0; w < vw; w++)
               d[(L * ns2 + size_t(k) * r[L] + n) * vw + w] = d[(L * ns2 + size_t(k) * ns + n) * vw + w];
     };
-    std::vector<double> h;
+    // complex operators: each level's K x N matrix (K rows of N complex
+    // entries, row stride N) -> real 2K x 2N with [[Re, Im], [-Im, Re]]
+    // blocks, levels at a stride of 4 ns^2 doubles (see gemm_op_size)
+    auto expand = [&](const std::vector<double>& d, int K, const std::vector<int>* rk, int N, std::vector<double>& out) {
+      const size_t src_level = size_t(K) * N * 2, dst_level = size_t(K) * N * 4;
+      out.assign(size_t(nlevels) * dst_level, 0.0);
+      for (int L = 0; L < nlevels; L++) {
+        const int n = rk ? (*rk)[L] : N;
+        const double* src = d.data() + size_t(L) * src_level;
+        double* dst = out.data() + size_t(L) * dst_level;
+        for (int k = 0; k < K; k++)
+          for (int j = 0; j < n; j++) {
+            const double re = src[(size_t(k) * n + j) * 2], im = src[(size_t(k) * n + j) * 2 + 1];
+            dst[size_t(2 * k) * (2 * n) + 2 * j] = re;
+            dst[size_t(2 * k) * (2 * n) + 2 * j + 1] = im;
+            dst[size_t(2 * k + 1) * (2 * n) + 2 * j] = -im;
+            dst[size_t(2 * k + 1) * (2 * n) + 2 * j + 1] = re;
+          }
+      }
+    };
+    std::vector<double> h, hx;
+    auto put = [&](DevBuf<double>& dst, const std::vector<int>* rk) {
+      if (!c->cplx) {
+        dst.upload(h, c->stream);
+        return;
+      }
+      expand(h, ns, rk, ns, hx);
+      dst.upload(hx, c->stream);
+      HIPCHK(hipStreamSynchronize(c->stream));  // hx is reused
+    };
     transpose_set(uc2e_u, h);
     compact(h, c->r_up);
-    c->uc2e_uT.upload(h, c->stream);
+    put(c->uc2e_uT, &c->r_up);
     transpose_set(uc2e_v, h);
-    c->uc2e_vT.upload(h, c->stream);
+    put(c->uc2e_vT, nullptr);
     transpose_set(dc2e_u, h);
     compact(h, c->r_dn);
-    c->dc2e_uT.upload(h, c->stream);
+    put(c->dc2e_uT, &c->r_dn);
     transpose_set(dc2e_v, h);
-    c->dc2e_vT.upload(h, c->stream);
+    put(c->dc2e_vT, nullptr);
     // M2Mcat[L]: (8 ns) x ns, row o*ns + k, col n = M2M[L][o][n][k]
     // L2Lcat[L]: ns x (8 ns), row k, col o*ns + n = L2L[L][o][n][k]
     std::vector<double> mm(size_t(nlevels) * 8 * ns2 * vw), ll(size_t(nlevels) * 8 * ns2 * vw);
@@ -689,15 +728,25 @@ int xfmm_set_operators(xfmm_context* c, int nlevels, const double* uc2e_u, const
               mm[((size_t(L) * 8 * ns2) + (size_t(o) * ns + k) * ns + n) * vw + w] = m2m[s];
               ll[((size_t(L) * 8 * ns2) + size_t(k) * 8 * ns + size_t(o) * ns + n) * vw + w] = l2l[s];
             }
-    c->m2mcat.upload(mm, c->stream);
-    c->l2lcat.upload(ll, c->stream);
+    if (c->cplx) {
+      // M2Mcat: (8 ns) x ns complex -> (16 ns) x (2 ns); L2Lcat: ns x (8 ns) -> (2 ns) x (16 ns)
+      expand(mm, 8 * ns, nullptr, ns, hx);
+      c->m2mcat.upload(hx, c->stream);
+      HIPCHK(hipStreamSynchronize(c->stream));
+      expand(ll, ns, nullptr, 8 * ns, hx);
+      c->l2lcat.upload(hx, c->stream);
+      HIPCHK(hipStreamSynchronize(c->stream));
+    } else {
+      c->m2mcat.upload(mm, c->stream);
+      c->l2lcat.upload(ll, c->stream);
+    }
     // FP32 copies of the GEMM operators for the FP32 translation GEMMs
     const std::pair<DevBuf<double>*, DevBuf<float>*> f32ops[] = {
         {&c->uc2e_uT, &c->uc2e_uT32}, {&c->uc2e_vT, &c->uc2e_vT32}, {&c->dc2e_uT, &c->dc2e_uT32},
         {&c->dc2e_vT, &c->dc2e_vT32}, {&c->m2mcat, &c->m2mcat32},   {&c->l2lcat, &c->l2lcat32}};
     for (int i = 0; i < 6; i++) {
       const auto& pr = f32ops[i];
-      if ((i < 4 ? c->f32_pinv : c->f32_trans) && !c->cplx) {
+      if (i < 4 ? c->f32_pinv : c->f32_trans) {
         pr.second->alloc(pr.first->n);
         launch_to_float(pr.first->ptr, pr.second->ptr, pr.first->n, c->stream);
       } else {
@@ -862,8 +911,8 @@ int xfmm_set_tree(xfmm_context* c, const xfmm_tree* t) {
     c->leaf_level_off[D + 1] = int(leaf.size());
     std::vector<long> lsr(leaf.size()), lnr(leaf.size());
     for (size_t i = 0; i < leaf.size(); i++) {
-      lsr[i] = long(i) * c->nsurf;
-      lnr[i] = long(leaf[i]) * c->nsurf;
+      lsr[i] = long(i) * c->nsurf * c->vw;  // row offsets of the GEMMs, in doubles
+      lnr[i] = long(leaf[i]) * c->nsurf * c->vw;
     }
     auto tiles = [&](const std::vector<int>& w) {
       std::vector<int> v;
@@ -898,8 +947,8 @@ int xfmm_set_tree(xfmm_context* c, const xfmm_tree* t) {
       c->nl_level_off[l] = int(nr.size());
       for (int i = lo[l]; i < lo[l + 1]; i++)
         if (ch[i] >= 0) {
-          nr.push_back(long(i) * c->nsurf);
-          ncr.push_back(long(ch[i]) * c->nsurf);
+          nr.push_back(long(i) * c->nsurf * c->vw);
+          ncr.push_back(long(ch[i]) * c->nsurf * c->vw);
         }
     }
     c->nl_level_off[D + 1] = int(nr.size());
@@ -1052,9 +1101,9 @@ int xfmm_set_tree(xfmm_context* c, const xfmm_tree* t) {
       size_t need = 0;  // split-K scratch of the per-level M2M / L2L GEMMs
       for (int l = 0; l < D; l++) {
         const int rows = c->nl_level_off[l + 1] - c->nl_level_off[l];
-        if (rows <= 0 || c->cplx) continue;
-        need = std::max(need, gemm_scratch_doubles(rows, c->nsurf, 8 * c->nsurf));
-        need = std::max(need, gemm_scratch_doubles(rows, 8 * c->nsurf, c->nsurf));
+        if (rows <= 0) continue;
+        need = std::max(need, gemm_scratch_doubles(rows, c->nsurf * c->vw, 8 * c->nsurf * c->vw));
+        need = std::max(need, gemm_scratch_doubles(rows, 8 * c->nsurf * c->vw, c->nsurf * c->vw));
       }
       c->gemm_scratch.alloc(need);
     }

@@ -10,84 +10,16 @@
 // (fmm_scale_invariant.h:260-268: N = 8*nsurf against the concatenated
 // transposed L2L matrices, accumulating into the contiguous child vectors).
 //
-// Real operands run on FP64 MFMA (k_gemm_rows_mfma64, split-K when the output
-// has too few tiles to fill the chip).  Complex operands (Helmholtz) use a
-// VALU kernel: 256 threads, 64 x 64 tile, 4 x 4 complex outputs per thread,
-// K staged through LDS in steps of 8.
+// All of them run on the matrix cores (k_gemm_rows_mfma64, split-K when the
+// output has too few tiles to fill the chip).  Complex operands (Helmholtz)
+// come as real GEMMs: interleaved (re, im) rows times the operator expanded
+// into real 2 x 2 blocks [[Re, Im], [-Im, Re]] (K and N doubled, built once
+// by the engine when the operators are set).
 #include <stdexcept>
 
 #include "kernels.h"
 
 namespace xfmm {
-
-// complex: values are interleaved (re, im); offsets are in complex elements
-template <int BM, int BN, int TM, int TN, int BK>
-__global__ __launch_bounds__(256) void k_gemm_rows_cplx(int m, int n, int k, const double2* __restrict__ A,
-                                                       const long* __restrict__ arow, const double2* __restrict__ B,
-                                                       double2* __restrict__ C, const long* __restrict__ crow,
-                                                       double alpha, int beta) {
-  static_assert((BM / TM) * (BN / TN) == 256, "thread layout");
-  __shared__ double2 As[BK][BM + 1];
-  __shared__ double2 Bs[BK][BN];
-  __shared__ long arow_s[BM];
-  const int tid = threadIdx.x;
-  const int row0 = blockIdx.y * BM;
-  const int col0 = blockIdx.x * BN;
-  for (int r = tid; r < BM; r += 256) arow_s[r] = (row0 + r < m) ? arow[row0 + r] : -1;
-  const int tr = tid / (BN / TN);
-  const int tc = tid % (BN / TN);
-  double2 acc[TM][TN];
-#pragma unroll
-  for (int i = 0; i < TM; i++)
-#pragma unroll
-    for (int j = 0; j < TN; j++) acc[i][j] = make_double2(0.0, 0.0);
-  __syncthreads();
-  for (int k0 = 0; k0 < k; k0 += BK) {
-    for (int e = tid; e < BM * BK; e += 256) {
-      const int r = e / BK, kk = e % BK;
-      const long ar = arow_s[r];
-      As[kk][r] = (ar >= 0 && k0 + kk < k) ? A[ar + k0 + kk] : make_double2(0.0, 0.0);
-    }
-    for (int e = tid; e < BK * BN; e += 256) {
-      const int kk = e / BN, c = e % BN;
-      Bs[kk][c] = (k0 + kk < k && col0 + c < n) ? B[size_t(k0 + kk) * n + col0 + c] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < BK; kk++) {
-      double2 a[TM], b[TN];
-#pragma unroll
-      for (int i = 0; i < TM; i++) a[i] = As[kk][tr * TM + i];
-#pragma unroll
-      for (int j = 0; j < TN; j++) b[j] = Bs[kk][tc * TN + j];
-#pragma unroll
-      for (int i = 0; i < TM; i++)
-#pragma unroll
-        for (int j = 0; j < TN; j++) {
-          acc[i][j].x = fma(a[i].x, b[j].x, acc[i][j].x);
-          acc[i][j].x = fma(-a[i].y, b[j].y, acc[i][j].x);
-          acc[i][j].y = fma(a[i].x, b[j].y, acc[i][j].y);
-          acc[i][j].y = fma(a[i].y, b[j].x, acc[i][j].y);
-        }
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int i = 0; i < TM; i++) {
-    const int r = row0 + tr * TM + i;
-    if (r >= m) continue;
-    const long cr = crow[r];
-    const double al = alpha;
-#pragma unroll
-    for (int j = 0; j < TN; j++) {
-      const int c = col0 + tc * TN + j;
-      if (c >= n) continue;
-      double2 v = make_double2(al * acc[i][j].x, al * acc[i][j].y);
-      if (beta) { v.x += C[cr + c].x; v.y += C[cr + c].y; }
-      C[cr + c] = v;
-    }
-  }
-}
 
 // MFMA path: BM x BN output tile per 256-thread workgroup, the 4 waves as
 // 2 x 2 sub-tiles of (BM/2) x (BN/2) (a grid of 16 x 16 MFMA tiles each), K
@@ -336,22 +268,13 @@ static void launch_real(int m, int n, int k, const double* A, const long* arow, 
   }
 }
 
-void launch_gemm_rows(bool cplx, bool f32, int m, int n, int k, const double* A, const long* arow, const void* Bv,
-                      double* C, const long* crow, double alpha, int beta, hipStream_t s, double* scratch,
-                      size_t scratch_doubles) {
-  const double* B = static_cast<const double*>(Bv);
+void launch_gemm_rows(bool f32, int m, int n, int k, const double* A, const long* arow, const void* Bv, double* C,
+                      const long* crow, double alpha, int beta, hipStream_t s, double* scratch, size_t scratch_doubles) {
   if (m <= 0 || n <= 0) return;
-  if (!cplx) {
-    if (f32) launch_real<float>(m, n, k, A, arow, static_cast<const float*>(Bv), C, crow, alpha, beta, s, scratch,
-                                scratch_doubles);
-    else launch_real<double>(m, n, k, A, arow, B, C, crow, alpha, beta, s, scratch, scratch_doubles);
-  } else {
-    constexpr int BM = 64, BN = 64;
-    dim3 grid((n + BN - 1) / BN, (m + BM - 1) / BM);
-    hipLaunchKernelGGL((k_gemm_rows_cplx<BM, BN, 4, 4, 8>), grid, dim3(256), 0, s, m, n, k,
-                       reinterpret_cast<const double2*>(A), arow, reinterpret_cast<const double2*>(B),
-                       reinterpret_cast<double2*>(C), crow, alpha, beta);
-  }
+  if (f32) launch_real<float>(m, n, k, A, arow, static_cast<const float*>(Bv), C, crow, alpha, beta, s, scratch,
+                              scratch_doubles);
+  else launch_real<double>(m, n, k, A, arow, static_cast<const double*>(Bv), C, crow, alpha, beta, s, scratch,
+                           scratch_doubles);
 }
 
 __global__ void k_zero(double* __restrict__ p, size_t n) {

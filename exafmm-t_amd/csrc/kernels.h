@@ -64,17 +64,16 @@ void launch_direct(int kind, bool mixed, bool cplx, Wavek k, const double* src, 
                    hipStream_t s);
 
 // C[crow[i]] (beta=0: =, beta=1: +=) alpha * A[arow[i]] x B for i < m.
-// A rows have K values, B is K x N row-major, C rows have N values (kind T).
-// arow/crow are row offsets in values (not rows); alpha is a power of two.
-// Real GEMMs run on FP64 MFMA, or with f32 on FP32 MFMA (operands rounded to
-// FP32 while staged, FP32 accumulation, FP64 results); when the output has few
-// tiles the K range is split over workgroups, partial products go to
-// `scratch` (at least gemm_scratch_doubles(m, n, k) doubles) and are reduced
-// in a fixed order.  B is float (f32 and real) or double; complex GEMMs are
-// FP64 (f32 ignored).
-void launch_gemm_rows(bool cplx, bool f32, int m, int n, int k, const double* A, const long* arow, const void* B,
-                      double* C, const long* crow, double alpha, int beta, hipStream_t s,
-                      double* scratch = nullptr, size_t scratch_doubles = 0);
+// A rows have K values, B is K x N row-major, C rows have N values (doubles;
+// arow/crow are row offsets in doubles, not rows); alpha is a power of two.
+// FP64 MFMA, or with f32 FP32 MFMA (operands rounded to FP32 while staged,
+// FP32 accumulation, FP64 results; B float); when the output has few tiles
+// the K range is split over workgroups, partial products go to `scratch` (at
+// least gemm_scratch_doubles(m, n, k) doubles) and are reduced in a fixed
+// order.  Complex GEMMs are called as real ones on the 2 x 2 block form.
+void launch_gemm_rows(bool f32, int m, int n, int k, const double* A, const long* arow, const void* B, double* C,
+                      const long* crow, double alpha, int beta, hipStream_t s, double* scratch = nullptr,
+                      size_t scratch_doubles = 0);
 // out[i] = float(in[i]) for i < n
 void launch_to_float(const double* in, float* out, size_t n, hipStream_t s);
 size_t gemm_scratch_doubles(int m, int n, int k);

@@ -20,7 +20,6 @@
 // sources overlap in L2) and 16 frequencies; each wave takes 4 frequencies,
 // lanes are targets, the 8x8 matrix M[r][k] is wave-uniform (scalar loads)
 // and the 26 relative positions are accumulated in registers.
-#include <cstdlib>
 #include <type_traits>
 
 #include "kernels.h"
@@ -648,171 +647,6 @@ __global__ __launch_bounds__(256) void k_hadamard_h(M2LDev m, const h8* __restri
   }
 }
 
-// Split-FP16 Hadamard, one wave per workgroup and per SIMD: the wave owns a
-// whole block (<= 64 targets = 4 tiles of 16) and a chunk of kch <= 64
-// frequencies.  Same operands, scales, union slots and result layout as
-// k_hadamard_h, but the A operands (operators) live in registers and each
-// serves the 4 target tiles, so LDS carries only the source spectra (written
-// once per frequency, read once per tile and relative position).  Software
-// pipelined across frequencies: while frequency k computes from registers
-// (A) and LDS (B), the operators and the source union of frequency k + 1 are
-// already in flight into a second register set, so no global load is waited
-// for inside the MFMA sequence; with one wave per SIMD the registers hold both
-// sets.  One wave per workgroup needs no cross-wave barrier.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_hadamard_w(
-    M2LDev m, const h8* __restrict__ X, const float* __restrict__ hscale, HadBlocks hb, int nchunk, int kch) {
-  typedef float v4 __attribute__((ext_vector_type(4)));
-  typedef float f4 __attribute__((ext_vector_type(4)));
-  constexpr int US = HAD_UCAP + 1;
-  constexpr int NP = NM2L / 2;
-  constexpr int NT = 4;  // target tiles of 16
-  constexpr int NU_Q = (HAD_UCAP * 2 + 63) / 64;
-  __shared__ h8 Us[4 * US];  // [term][h][union slot]; slot HAD_UCAP stays zero
-  // raw FP32 union spectra of the next frequency, [2 q + half of the element][lane]:
-  // filled by global -> LDS loads (no registers held while they are in flight)
-  __shared__ f4 Rw[2 * NU_Q][64];
-  const int w = blockIdx.x;
-  const int x = w & 7, i = w >> 3;
-  const int chunk = (i / hb.nblk) * 8 + x;
-  const int b = i % hb.nblk;
-  if (chunk >= nchunk) return;
-  const int kbeg = chunk * kch, kend = min(m.nfreq, kbeg + kch);
-  const int lane = threadIdx.x;
-  const int j = lane & 15, g = lane >> 4;
-  const int nu = hb.nunion[b];
-  if (lane < 4) Us[lane * US + HAD_UCAP] = h8{0, 0, 0, 0, 0, 0, 0, 0};
-  // LDS element offsets of the lane's B operands, two 16-bit offsets per word
-  unsigned uo[NT][(NP + 1) / 2];
-#pragma unroll
-  for (int t = 0; t < NT; t++)
-#pragma unroll
-    for (int pm = 0; pm < NP; pm += 2) {
-      unsigned v = 0;
-#pragma unroll
-      for (int h = 0; h < 2 && pm + h < NP; h++) {
-        const int u = hb.loc[(size_t(b) * NM2L + 2 * (pm + h) + (g >> 1)) * 64 + t * 16 + j];
-        v |= unsigned((g & 1) * US + (u >= 0 ? u : HAD_UCAP)) << (16 * h);
-      }
-      uo[t][pm / 2] = v;
-    }
-  const float amax = __uint_as_float(*m.amax);
-  int ex = 0;
-  frexpf(amax, &ex);
-  if (!(amax <= 3.0e38f)) ex = 15;
-  const float sb = ldexpf(1.f, 15 - ex), isb = ldexpf(1.f, ex - 15);
-  const f4* fin = reinterpret_cast<const f4*>(m.fft_in);  // [slot][k][2 halves of 4 children]
-  C2<float>* fout = reinterpret_cast<C2<float>*>(m.fft_out);
-  // staging: element e = lane + 64 q is union slot e / 2, children 4 (e % 2) .. + 3
-  const int hh = lane & 1;
-  int usrc[NU_Q];
-#pragma unroll
-  for (int q = 0; q < NU_Q; q++) {
-    const int e = lane + 64 * q;
-    usrc[q] = (e < nu * 2) ? hb.uni[size_t(b) * HAD_UCAP + (e >> 1)] : -1;
-  }
-  int trg[NT];
-#pragma unroll
-  for (int t = 0; t < NT; t++) trg[t] = hb.trg[size_t(b) * 64 + t * 16 + j];
-  h8 a0[NP][2], a1[NP][2];
-  float s0 = 0, s1 = 0;
-  // result scales of the chunk (kch <= 64): lane l holds frequency kbeg + l;
-  // read back with readlane, so no scalar memory load is in flight in the
-  // loop (a pending one would force every LDS wait to drain completely)
-  const float hs_l = (kbeg + lane < kend) ? hscale[kbeg + lane] : 0.f;
-  // operators, union spectra and result scale of frequency k into registers
-  auto fetch = [&](int k, h8 (&a)[NP][2], float& sc) {
-    const h8* xk = X + size_t(k) * NP * 2 * 64 + lane;
-#pragma unroll
-    for (int pm = 0; pm < NP; pm++) {
-      a[pm][0] = xk[(2 * pm) * 64];
-      a[pm][1] = xk[(2 * pm + 1) * 64];
-    }
-#pragma unroll
-    for (int q = 0; q < NU_Q; q++)
-      if (usrc[q] >= 0) {
-        const f4* f = fin + ((size_t(usrc[q]) * m.nfreq + k) * 2 + hh) * 2;
-        __builtin_amdgcn_global_load_lds(f, &Rw[2 * q][0], 16, 0, 0);
-        __builtin_amdgcn_global_load_lds(f + 1, &Rw[2 * q + 1][0], 16, 0, 0);
-      }
-    sc = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(hs_l), k - kbeg)) * isb;
-  };
-  // union spectra of the fetched frequency -> LDS as split FP16
-  auto stage = [&]() {
-    // the global -> LDS loads of the fetch (and anything else in flight)
-    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < NU_Q; q++)
-      if (usrc[q] >= 0) {
-        const f4 ra = Rw[2 * q][lane], rb = Rw[2 * q + 1][lane];
-        h8 hi, lo;
-#pragma unroll
-        for (int e = 0; e < 8; e++) {
-          const float v = sb * (e < 4 ? ra[e] : rb[e - 4]);
-          const _Float16 h = _Float16(v);
-          hi[e] = h;
-          lo[e] = _Float16(v - float(h));
-        }
-        const int slot = (lane + 64 * q) >> 1;
-        Us[hh * US + slot] = hi;
-        Us[2 * US + hh * US + slot] = lo;
-      }
-  };
-  // frequency k from operators a (LDS holds its union); the next frequency's
-  // data is fetched into an / its union staged once k has read the LDS
-  auto step = [&](int k, h8 (&a)[NP][2], float sc, h8 (&an)[NP][2], float& scn) {
-    __syncthreads();  // one wave: orders the staging writes before the reads
-    const bool more = k + 1 < kend;
-    if (more) fetch(k + 1, an, scn);
-    v4 ahh[NT], acr[NT];
-#pragma unroll
-    for (int t = 0; t < NT; t++) {
-      ahh[t] = v4{0, 0, 0, 0};
-      acr[t] = v4{0, 0, 0, 0};
-    }
-    // B operands read from LDS RD steps (tile, position pair) ahead of their
-    // MFMAs, so the LDS latency overlaps the matrix work
-    constexpr int RD = 8, NS = NP * NT;
-    h8 rbh[RD], rbl[RD];
-    auto bload = [&](int st, h8& bh, h8& bl) {
-      const int pm = st / NT, t = st % NT;
-      const int off = int((uo[t][pm / 2] >> (16 * (pm & 1))) & 0xffffu);
-      bh = Us[off];
-      bl = Us[2 * US + off];
-    };
-#pragma unroll
-    for (int st = 0; st < RD; st++) bload(st, rbh[st], rbl[st]);
-    // schedule: the RD steps' LDS reads first, then per step its 3 MFMAs
-    // followed by the 2 reads of step st + RD (keeps RD steps of reads in flight)
-    __builtin_amdgcn_sched_group_barrier(0x100, 2 * RD, 0);
-#pragma unroll
-    for (int st = 0; st < NS; st++) {
-      const int pm = st / NT, t = st % NT;
-      const h8 bh = rbh[st % RD], bl = rbl[st % RD];
-      if (st + RD < NS) bload(st + RD, rbh[st % RD], rbl[st % RD]);
-      ahh[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[pm][0], bh, ahh[t], 0, 0, 0);
-      acr[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[pm][0], bl, acr[t], 0, 0, 0);
-      acr[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[pm][1], bh, acr[t], 0, 0, 0);
-      __builtin_amdgcn_sched_group_barrier(0x008, 3, 0);
-      if (st + RD < NS) __builtin_amdgcn_sched_group_barrier(0x100, 2, 0);
-    }
-#pragma unroll
-    for (int t = 0; t < NT; t++)
-      if (trg[t] >= 0)
-        *reinterpret_cast<v4*>(fout + (size_t(trg[t]) * m.nfreq + k) * NCHILD + 2 * g) = (ahh[t] + acr[t]) * sc;
-    if (more) {
-      __syncthreads();  // the reads of frequency k before the next staging
-      stage();
-    }
-  };
-  fetch(kbeg, a0, s0);
-  stage();
-  for (int k = kbeg; k < kend; k += 2) {
-    step(k, a0, s0, a1, s1);
-    if (k + 1 < kend) step(k + 1, a1, s1, a0, s0);
-  }
-}
-
 // ------------------------------------------------ per-node MFMA transforms
 //
 // One workgroup per M2L node (its 8 children), in kx slabs.  Each 1-D DFT
@@ -1227,14 +1061,7 @@ void launch_hadamard(const M2LDev& m, const void* m2l_opsx, const float* hscale,
   const int nchunk = (m.nfreq + kch - 1) / kch;
   const int nchunk8 = (nchunk + 7) / 8 * 8;
   dim3 grid(unsigned(nchunk8) * unsigned(hb.nblk));
-  static const int had_variant = [] {
-    const char* e = std::getenv("XFMM_HAD_VARIANT");
-    return e ? std::atoi(e) : 0;
-  }();
-  if (m.mixed && hscale && m.amax && had_variant == 1 && kch <= 64)
-    hipLaunchKernelGGL(k_hadamard_w, grid, dim3(64), 0, s, m, static_cast<const h8*>(m2l_opsx), hscale, hb, nchunk,
-                       kch);
-  else if (m.mixed && hscale && m.amax)
+  if (m.mixed && hscale && m.amax)
     hipLaunchKernelGGL(k_hadamard_h, grid, dim3(256), 0, s, m, static_cast<const h8*>(m2l_opsx), hscale, hb, nchunk,
                        kch);
   else if (m.mixed)

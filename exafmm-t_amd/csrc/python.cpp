@@ -47,6 +47,7 @@ Bodies<T> init_sources(py::array_t<real_t, py::array::c_style | py::array::force
   Bodies<T> b(n);
   auto c = coords.template unchecked<2>();
   auto q = charges.template unchecked<1>();
+#pragma omp parallel for schedule(static)
   for (ssize_t i = 0; i < n; i++) {
     for (int d = 0; d < 3; d++) b[i].X[d] = c(i, d);
     b[i].q = q(i);
@@ -61,6 +62,7 @@ Bodies<T> init_targets(py::array_t<real_t, py::array::c_style | py::array::force
   const ssize_t n = coords.shape(0);
   Bodies<T> b(n);
   auto c = coords.template unchecked<2>();
+#pragma omp parallel for schedule(static)
   for (ssize_t i = 0; i < n; i++) {
     for (int d = 0; d < 3; d++) b[i].X[d] = c(i, d);
     b[i].ibody = int(i);

@@ -15,39 +15,61 @@
 
 namespace exafmm_t {
 
+namespace detail {
+
+// The colleagues of one node from those of its parent (which must be set).
 template <typename T>
-void set_colleagues(Node<T>* node) {
+void colleagues_of(Node<T>* node) {
   node->colleagues.assign(27, nullptr);
   if (node->level == 0) {
     node->colleagues[13] = node;
-  } else {
-    const int o = node->octant;
-    for (int i = 0; i < 27; i++) {
-      Node<T>* pc = node->parent->colleagues[i];
-      if (!pc || pc->is_leaf) continue;
-      for (int j = 0; j < NCHILD; j++) {
-        Node<T>* ch = pc->children[j];
-        if (!ch) continue;
-        int slot = 0, stride = 1;
-        bool ok = true;
-        for (int d = 0; d < 3; d++) {
-          const int pd = (i / stride) % 3 - 1;                       // parent colleague offset
-          const int diff = 2 * pd + ((j >> d) & 1) - ((o >> d) & 1);  // child offset
-          if (diff < -1 || diff > 1) ok = false;
-          slot += (diff + 1) * stride;
-          stride *= 3;
-        }
-        if (ok) node->colleagues[slot] = ch;
+    return;
+  }
+  const int o = node->octant;
+  for (int i = 0; i < 27; i++) {
+    Node<T>* pc = node->parent->colleagues[i];
+    if (!pc || pc->is_leaf) continue;
+    for (int j = 0; j < NCHILD; j++) {
+      Node<T>* ch = pc->children[j];
+      if (!ch) continue;
+      int slot = 0, stride = 1;
+      bool ok = true;
+      for (int d = 0; d < 3; d++) {
+        const int pd = (i / stride) % 3 - 1;                       // parent colleague offset
+        const int diff = 2 * pd + ((j >> d) & 1) - ((o >> d) & 1);  // child offset
+        if (diff < -1 || diff > 1) ok = false;
+        slot += (diff + 1) * stride;
+        stride *= 3;
       }
+      if (ok) node->colleagues[slot] = ch;
     }
   }
+}
+
+}  // namespace detail
+
+// Colleagues of node and of its subtree, top-down.
+template <typename T>
+void set_colleagues(Node<T>* node) {
+  detail::colleagues_of(node);
   if (!node->is_leaf)
     for (int c = 0; c < NCHILD; c++)
       if (node->children[c]) set_colleagues(node->children[c]);
 }
 
+// Colleagues of every node, level by level (a node reads only its parent's
+// colleagues, so the nodes of one level are independent).
 template <typename T>
-void set_colleagues(Nodes<T>& nodes) { set_colleagues(&nodes[0]); }
+void set_colleagues(Nodes<T>& nodes) {
+  int maxl = 0;
+  for (auto& n : nodes) maxl = std::max(maxl, n.level);
+  std::vector<std::vector<Node<T>*>> lev(maxl + 1);
+  for (auto& n : nodes) lev[n.level].push_back(&n);
+  for (auto& l : lev) {
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long i = 0; i < long(l.size()); i++) detail::colleagues_of(l[i]);
+  }
+}
 
 namespace detail {
 

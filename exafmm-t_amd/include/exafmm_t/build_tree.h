@@ -84,8 +84,22 @@ void get_bounds(const Bodies<T>& sources, const Bodies<T>& targets, vec3& x0, re
   if (sources.empty() && targets.empty()) throw std::runtime_error("get_bounds: no bodies");
   vec3 lo = sources.empty() ? targets[0].X : sources[0].X;
   vec3 hi = lo;
-  for (const auto& b : sources) { lo = min(lo, b.X); hi = max(hi, b.X); }
-  for (const auto& b : targets) { lo = min(lo, b.X); hi = max(hi, b.X); }
+  const long ns = long(sources.size()), nt = long(targets.size());
+#pragma omp parallel
+  {
+    vec3 tlo = lo, thi = hi;
+#pragma omp for nowait schedule(static)
+    for (long i = 0; i < ns + nt; i++) {
+      const vec3& x = i < ns ? sources[i].X : targets[i - ns].X;
+      tlo = min(tlo, x);
+      thi = max(thi, x);
+    }
+#pragma omp critical
+    {
+      lo = min(lo, tlo);
+      hi = max(hi, thi);
+    }
+  }
   x0 = (hi + lo) / 2;
   r0 = std::max(max(x0 - lo), max(hi - x0));
   r0 *= 1.00001;

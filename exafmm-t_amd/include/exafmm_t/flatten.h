@@ -149,18 +149,34 @@ FlatTree<T> flatten(Nodes<T>& nodes, int depth, real_t r0, const vec3& x0) {
   f.src_count.assign(nn, 0);
   f.trg_begin.assign(nn, 0);
   f.trg_count.assign(nn, 0);
-  for (int i : leaves) {
+  const int nl = int(leaves.size());
+  f.leaf_order.resize(nl);
+  int ns_tot = 0, nt_tot = 0;
+  for (int k = 0; k < nl; k++) {
+    const int i = leaves[k];
     Node<T>* n = f.dev2node[i];
-    f.leaf_order.push_back(n);
-    const int ns = int(n->src_coord.size() / 3), nt = int(n->trg_coord.size() / 3);
-    f.src_begin[i] = int(f.src_ibody.size());
-    f.src_count[i] = ns;
-    f.trg_begin[i] = int(f.trg_ibody.size());
-    f.trg_count[i] = nt;
-    f.src_coord.insert(f.src_coord.end(), n->src_coord.begin(), n->src_coord.end());
-    f.trg_coord.insert(f.trg_coord.end(), n->trg_coord.begin(), n->trg_coord.end());
-    for (int j = 0; j < ns; j++) f.src_ibody.push_back(j < int(n->isrcs.size()) ? n->isrcs[j] : -1);
-    for (int j = 0; j < nt; j++) f.trg_ibody.push_back(j < int(n->itrgs.size()) ? n->itrgs[j] : -1);
+    f.leaf_order[k] = n;
+    f.src_begin[i] = ns_tot;
+    f.src_count[i] = int(n->src_coord.size() / 3);
+    f.trg_begin[i] = nt_tot;
+    f.trg_count[i] = int(n->trg_coord.size() / 3);
+    ns_tot += f.src_count[i];
+    nt_tot += f.trg_count[i];
+  }
+  f.src_coord.resize(3 * size_t(ns_tot));
+  f.trg_coord.resize(3 * size_t(nt_tot));
+  f.src_ibody.resize(ns_tot);
+  f.trg_ibody.resize(nt_tot);
+  // leaves own disjoint ranges: copy them in parallel
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int k = 0; k < nl; k++) {
+    const int i = leaves[k];
+    Node<T>* n = f.dev2node[i];
+    const int ns = f.src_count[i], nt = f.trg_count[i], sb = f.src_begin[i], tb = f.trg_begin[i];
+    std::copy(n->src_coord.begin(), n->src_coord.end(), f.src_coord.begin() + 3 * size_t(sb));
+    std::copy(n->trg_coord.begin(), n->trg_coord.end(), f.trg_coord.begin() + 3 * size_t(tb));
+    for (int j = 0; j < ns; j++) f.src_ibody[sb + j] = j < int(n->isrcs.size()) ? n->isrcs[j] : -1;
+    for (int j = 0; j < nt; j++) f.trg_ibody[tb + j] = j < int(n->itrgs.size()) ? n->itrgs[j] : -1;
   }
   // lists
   auto csr = [&](auto getter, std::vector<int>& off, std::vector<int>& lst) {
