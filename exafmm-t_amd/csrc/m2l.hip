@@ -210,19 +210,21 @@ __global__ __launch_bounds__(256) void k_ifft_dn(M2LDev m, DevTree t, const int*
 // its p (or 2p) inputs once and produces all outputs of the line.
 
 // LDS of the per-node MFMA transforms (rb = bytes of the compute type):
-// forward: twiddles, the 8 children's [0,p)^3 input grids, y-pass input
-// [2p][8p] and z-pass input [2p][16p]; inverse: twiddles, y-pass input
-// [4p][8p] and the y-pass output [p][p][8] complex.
+// forward: twiddles, the 8 children's [0,p)^3 input grids (complex for
+// Helmholtz), y-pass input [2p][8p] and z-pass input [2p][16p]; inverse:
+// twiddles, y-pass input [4p][8p] and the y-pass output [p][p][8] complex.
 static size_t fft_node_lds(const M2LDev& m, size_t rb) {
-  const size_t p = m.p, n1 = m.n1;
-  const size_t fwd = n1 * 2 * rb + NCHILD * p * p * p * rb + 2 * p * 8 * p * rb + 2 * p * 16 * p * rb;
+  const size_t p = m.p, n1 = m.n1, vw = m.cplx ? 2 : 1;
+  const size_t fwd = n1 * 2 * rb + NCHILD * p * p * p * vw * rb + 2 * p * 8 * p * rb + 2 * p * 16 * p * rb;
   const size_t inv = n1 * 2 * rb + 4 * p * 8 * p * rb + p * p * NCHILD * 2 * rb;
   return fwd > inv ? fwd : inv;
 }
 
-// per-node transforms: real kernels, p in [3, 12], LDS within the CU's 160 KiB
+// per-node transforms: p in [3, 12], LDS within the CU's 160 KiB (the
+// complex FP64 forward grid at p = 10 is not: Helmholtz in FP64 keeps the
+// per-child transforms there)
 static bool fft_node_ok(const M2LDev& m) {
-  return !m.cplx && m.p >= 3 && m.p <= 12 && fft_node_lds(m, (m.mixed && m.mixed_fft) ? 4 : 8) <= 160 * 1024;
+  return m.p >= 3 && m.p <= 12 && fft_node_lds(m, (m.mixed && m.mixed_fft) ? 4 : 8) <= 160 * 1024;
 }
 
 template <typename F>
@@ -321,7 +323,8 @@ struct HadRow<double> {
 };
 
 template <typename R>
-__global__ void k_m2l_expand(int nlev, int nfreq, int n1, bool slab, const R* __restrict__ M, R* __restrict__ X) {
+__global__ void k_m2l_expand(int nlev, int nfreq, int n1, int nh, bool slab, const R* __restrict__ M,
+                             R* __restrict__ X) {
   const size_t n = size_t(nlev) * nfreq * NM2L * 64;
   for (size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += size_t(gridDim.x) * blockDim.x) {
     const int lane = int(e % 64), g = lane >> 4, j = lane & 15, t = HadRow<R>::child(j);
@@ -332,7 +335,7 @@ __global__ void k_m2l_expand(int nlev, int nfreq, int n1, bool slab, const R* __
     int f = k;
     if (slab) {
       const int kx = k / (n1 * n1), rem = k % (n1 * n1);
-      f = kx + (n1 / 2 + 1) * rem;  // rem = ky + n1 kz
+      f = kx + nh * rem;  // rem = ky + n1 kz
     }
     const R* src = M + ((size_t(L) * NM2L + r) * nfreq + f) * 128;
     const R m0x = src[2 * (g * 8 + t)], m0y = src[2 * (g * 8 + t) + 1];
@@ -354,14 +357,15 @@ __global__ void k_m2l_expand(int nlev, int nfreq, int n1, bool slab, const R* __
 // [2^14, 2^15), and split a = hi + lo, hi = fp16(a), lo = fp16(a - hi).
 // Stored as X[((L nfreq + k) 13 + pm) 2 + term][lane][8] (16 bytes per lane);
 // hscale[L][k] = 2^-e(L,k).
-__global__ __launch_bounds__(256) void k_m2l_expand_h(int nfreq, int n1, bool slab, const float* __restrict__ M,
-                                                      h8* __restrict__ X, float* __restrict__ hscale) {
+__global__ __launch_bounds__(256) void k_m2l_expand_h(int nfreq, int n1, int nh, bool slab,
+                                                      const float* __restrict__ M, h8* __restrict__ X,
+                                                      float* __restrict__ hscale) {
   __shared__ float red[256];
   const int lk = blockIdx.x, k = lk % nfreq, L = lk / nfreq, tid = threadIdx.x;
   int f = k;
   if (slab) {
     const int kx = k / (n1 * n1), rem = k % (n1 * n1);
-    f = kx + (n1 / 2 + 1) * rem;
+    f = kx + nh * rem;
   }
   auto src = [&](int r) { return M + ((size_t(L) * NM2L + r) * nfreq + f) * 128; };
   float mx = 0.f;
@@ -402,15 +406,18 @@ __global__ __launch_bounds__(256) void k_m2l_expand_h(int nfreq, int n1, bool sl
 void launch_m2l_expand(const M2LDev& m, int nlev, const void* M, void* X, float* hscale, hipStream_t s) {
   const size_t n = size_t(nlev) * m.nfreq * NM2L * 64;
   const unsigned blocks = unsigned(std::min<size_t>((n + 255) / 256, 65536));
+  // the per-node transforms store frequencies kx-slab major: k = kx n1^2 +
+  // ky + n1 kz, reference f = kx + nh (ky + n1 kz)
   const bool slab = fft_node_ok(m);
+  const int nh = m.cplx ? m.n1 : m.n1 / 2 + 1;
   if (m.mixed && hscale)
-    hipLaunchKernelGGL(k_m2l_expand_h, dim3(unsigned(nlev * m.nfreq)), dim3(256), 0, s, m.nfreq, m.n1, slab,
+    hipLaunchKernelGGL(k_m2l_expand_h, dim3(unsigned(nlev * m.nfreq)), dim3(256), 0, s, m.nfreq, m.n1, nh, slab,
                        static_cast<const float*>(M), static_cast<h8*>(X), hscale);
   else if (m.mixed)
-    hipLaunchKernelGGL(k_m2l_expand<float>, dim3(blocks), dim3(256), 0, s, nlev, m.nfreq, m.n1, slab,
+    hipLaunchKernelGGL(k_m2l_expand<float>, dim3(blocks), dim3(256), 0, s, nlev, m.nfreq, m.n1, nh, slab,
                        static_cast<const float*>(M), static_cast<float*>(X));
   else
-    hipLaunchKernelGGL(k_m2l_expand<double>, dim3(blocks), dim3(256), 0, s, nlev, m.nfreq, m.n1, slab,
+    hipLaunchKernelGGL(k_m2l_expand<double>, dim3(blocks), dim3(256), 0, s, nlev, m.nfreq, m.n1, nh, slab,
                        static_cast<const double*>(M), static_cast<double*>(X));
 }
 
@@ -687,14 +694,18 @@ __device__ __forceinline__ double dft_entry(int ph, int n1, int sign, int ro, in
   return ro == 0 ? (ri == 0 ? cs : -sn) : (ri == 0 ? sn : cs);
 }
 
-// fft_up_equiv for one node: out[(kx + nh (ky + n1 kz)) * 8 + c] = sum over
+// fft_up_equiv for one node: out[(kx n1^2 + ky + n1 kz) * 8 + c] = sum over
 // the child's surface points (x,y,z) of up[c][i] exp(-2 pi i (kx x + ky y +
-// kz z) / n1)  (reference fmm_scale_invariant.h:560-604).
-template <typename RS, typename R, int P>
+// kz z) / n1), kx over the r2c half axis [0, p] for the real kernels
+// (reference fmm_scale_invariant.h:560-604) and over [0, 2p) for Helmholtz,
+// whose densities are complex (c2c, fmm.h:631-711).
+template <typename RS, typename R, int P, bool CPLX>
 __global__ __launch_bounds__(256) void k_fft_up_mfma(M2LDev m, DevTree t, const int* __restrict__ src_node,
                                                      int nslots) {
   typedef typename MfmaT<R>::v4 v4;
   constexpr int n1 = 2 * P, nh = P + 1;
+  constexpr int NKX = CPLX ? n1 : nh;               // kx slabs
+  constexpr int VW = CPLX ? 2 : 1;                  // reals per density value
   constexpr int KK = 2 * P, KS = (KK + 3) / 4;      // inner: (y or z, re/im)
   constexpr int MT = (2 * n1 + 15) / 16;             // rows: (ky or kz, re/im)
   constexpr int NCY = 8 * P, NTY = (NCY + 15) / 16;  // y pass columns (z, c)
@@ -705,19 +716,25 @@ __global__ __launch_bounds__(256) void k_fft_up_mfma(M2LDev m, DevTree t, const 
   const int ns = m.nsurf;
   constexpr int P3 = P * P * P;
   C2<R>* w = reinterpret_cast<C2<R>*>(smem_raw);  // exp(-2 pi i m / n1)
-  R* gx = reinterpret_cast<R*>(w + n1);           // [c][z][y][x] input grid
-  R* Ay = gx + NCHILD * P3;                       // [(y, re/im)][(z, c)]
+  R* gx = reinterpret_cast<R*>(w + n1);           // [c][z][y][x] input grid (x VW reals)
+  R* Ay = gx + NCHILD * P3 * VW;                  // [(y, re/im)][(z, c)]
   R* Bz = Ay + KK * NCY;                          // [(z, re/im)][(ky, c)]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int j = lane & 15, g = lane >> 4;
   make_twiddles(w, n1, -1);
-  for (int e = tid; e < NCHILD * P3; e += 256) gx[e] = R(0);
+  for (int e = tid; e < NCHILD * P3 * VW; e += 256) gx[e] = R(0);
   __syncthreads();
   const int node = src_node[slot];
-  const double* up = t.up + size_t(t.child0[node]) * ns;
+  const double* up = t.up + size_t(t.child0[node]) * ns * VW;
   for (int e = tid; e < NCHILD * ns; e += 256) {
     const int c = e / ns, i = e - c * ns;
-    gx[c * P3 + m.map_up[i]] = R(up[e]);
+    if constexpr (CPLX) {
+      R* gp = gx + 2 * (c * P3 + m.map_up[i]);
+      gp[0] = R(up[2 * e]);
+      gp[1] = R(up[2 * e + 1]);
+    } else {
+      gx[c * P3 + m.map_up[i]] = R(up[e]);
+    }
   }
   R wf[MT][KS];
 #pragma unroll
@@ -730,19 +747,23 @@ __global__ __launch_bounds__(256) void k_fft_up_mfma(M2LDev m, DevTree t, const 
   __syncthreads();
   C2<RS>* out = reinterpret_cast<C2<RS>*>(m.fft_in) + size_t(slot) * m.nfreq * NCHILD;
   float mx = 0.f;
-  for (int kx = 0; kx < nh; kx++) {
+  for (int kx = 0; kx < NKX; kx++) {
     // x: Ay[(y, re/im)][(z, c)] = sum_x gx[c][z][y][x] exp(-2 pi i kx x / n1)
     C2<R> tw[P];
 #pragma unroll
     for (int x = 0; x < P; x++) tw[x] = w[(kx * x) % n1];
     for (int e = tid; e < NCHILD * P * P; e += 256) {
       const int c = e & 7, line = e >> 3, y = line / P, z = line % P;
-      const R* gl = gx + c * P3 + (z * P + y) * P;
+      const R* gl = gx + (c * P3 + (z * P + y) * P) * VW;
       C2<R> acc{0, 0};
 #pragma unroll
       for (int x = 0; x < P; x++) {
-        acc.x = fma(gl[x], tw[x].x, acc.x);
-        acc.y = fma(gl[x], tw[x].y, acc.y);
+        if constexpr (CPLX) {
+          cfma(acc, C2<R>{gl[2 * x], gl[2 * x + 1]}, tw[x]);
+        } else {
+          acc.x = fma(gl[x], tw[x].x, acc.x);
+          acc.y = fma(gl[x], tw[x].y, acc.y);
+        }
       }
       Ay[(2 * y) * NCY + z * 8 + c] = acc.x;
       Ay[(2 * y + 1) * NCY + z * 8 + c] = acc.y;
@@ -806,12 +827,15 @@ __global__ __launch_bounds__(256) void k_fft_up_mfma(M2LDev m, DevTree t, const 
 // w(kx) Re(in(kx,ky,kz)[c] exp(+2 pi i (kx x + ky y + kz z) / n1)) at the
 // child's check-surface points (x,y,z) in [p,2p)^3, w = 1 for kx = 0 and the
 // Nyquist plane, else 2 (c2r of the half spectrum; reference
-// fmm_scale_invariant.h:606-646).
-template <typename RS, typename R, int P>
+// fmm_scale_invariant.h:606-646).  Helmholtz (CPLX): the complex sum over
+// the full kx axis with w = 1 (c2c, fmm.h:713-788).
+template <typename RS, typename R, int P, bool CPLX>
 __global__ __launch_bounds__(256) void k_ifft_dn_mfma(M2LDev m, DevTree t, const int* __restrict__ trg_node,
                                                       const double* __restrict__ scale, int nslots) {
   typedef typename MfmaT<R>::v4 v4;
   constexpr int n1 = 2 * P, nh = P + 1;
+  constexpr int NKX = CPLX ? n1 : nh;                // kx slabs
+  constexpr int VW = CPLX ? 2 : 1;                   // reals per check potential
   constexpr int KK = 2 * n1;                         // inner: (kz or ky, re/im)
   constexpr int KS = 2 * ((n1 + 3) / 4);             // k-steps (pairs of steps share a frequency)
   constexpr int MT = (2 * P + 15) / 16;              // rows: (z or y in [p,2p), re/im)
@@ -842,11 +866,13 @@ __global__ __launch_bounds__(256) void k_ifft_dn_mfma(M2LDev m, DevTree t, const
   // x (c2r) results of this thread: the whole x line of y-pass outputs
   // e = tid + 256 r = (z, y, c) (Ap index e), i.e. the grid points x in
   // [p, 2p) of that line, accumulated over the kx slabs in registers
-  R gx[NL][P];
+  R gx[NL][P][VW];
 #pragma unroll
   for (int r = 0; r < NL; r++)
 #pragma unroll
-    for (int x = 0; x < P; x++) gx[r][x] = R(0);
+    for (int x = 0; x < P; x++)
+#pragma unroll
+      for (int v = 0; v < VW; v++) gx[r][x][v] = R(0);
   const C2<RS>* in = reinterpret_cast<const C2<RS>*>(m.fft_out) + size_t(slot) * m.nfreq * NCHILD;
   // z-pass input fragments of this wave's tiles, loaded one slab ahead
   RS bz[NTZW][KS];
@@ -866,7 +892,7 @@ __global__ __launch_bounds__(256) void k_ifft_dn_mfma(M2LDev m, DevTree t, const
   };
   fetch(0);
   __syncthreads();
-  for (int kx = 0; kx < nh; kx++) {
+  for (int kx = 0; kx < NKX; kx++) {
     // z: Bp[(ky, re/im)][(z, c)] = sum_kz W(z, kz) in(kx, ky, kz)[c]
 #pragma unroll
     for (int s = 0; s < NTZW; s++) {
@@ -888,7 +914,7 @@ __global__ __launch_bounds__(256) void k_ifft_dn_mfma(M2LDev m, DevTree t, const
         }
       }
     }
-    if (kx + 1 < nh) fetch(kx + 1);
+    if (kx + 1 < NKX) fetch(kx + 1);
     __syncthreads();
     // y: Ap[z][y][c] = sum_ky W(y, ky) Bp[(ky, .)][(z, c)]
     for (int nt = wave; nt < NTY; nt += 4) {
@@ -913,9 +939,9 @@ __global__ __launch_bounds__(256) void k_ifft_dn_mfma(M2LDev m, DevTree t, const
       }
     }
     __syncthreads();
-    // x (c2r) over each line, weight 1 for kx = 0 and the Nyquist plane,
-    // else 2 (the twiddles are wave-uniform)
-    const R wt = (kx == 0 || kx == nh - 1) ? R(1) : R(2);
+    // x over each line: c2r with weight 1 for kx = 0 and the Nyquist plane,
+    // else 2; c2c (Helmholtz) with weight 1 (the twiddles are wave-uniform)
+    const R wt = (CPLX || kx == 0 || kx == nh - 1) ? R(1) : R(2);
     C2<R> tw[P];
 #pragma unroll
     for (int x = 0; x < P; x++) {
@@ -928,7 +954,10 @@ __global__ __launch_bounds__(256) void k_ifft_dn_mfma(M2LDev m, DevTree t, const
       if (e < NCHILD * P * P) {
         const C2<R> a = Ap[e];
 #pragma unroll
-        for (int x = 0; x < P; x++) gx[r][x] += a.x * tw[x].x - a.y * tw[x].y;
+        for (int x = 0; x < P; x++) {
+          gx[r][x][0] += a.x * tw[x].x - a.y * tw[x].y;
+          if constexpr (CPLX) gx[r][x][VW - 1] += a.x * tw[x].y + a.y * tw[x].x;
+        }
       }
     }
     // the next z pass writes Bp only (its readers are behind the barrier
@@ -936,7 +965,7 @@ __global__ __launch_bounds__(256) void k_ifft_dn_mfma(M2LDev m, DevTree t, const
   }
   const int node = trg_node[slot];
   const double sc = scale[slot];
-  double* dn = t.dn + size_t(t.child0[node]) * ns;
+  double* dn = t.dn + size_t(t.child0[node]) * ns * VW;
   // scatter the surface points of the lines (each surface point is one
   // grid point of one line)
 #pragma unroll
@@ -947,7 +976,10 @@ __global__ __launch_bounds__(256) void k_ifft_dn_mfma(M2LDev m, DevTree t, const
 #pragma unroll
     for (int x = 0; x < P; x++) {
       const int i = m.inv_dn[zy * P + x];
-      if (i >= 0) dn[size_t(c) * ns + i] += sc * double(gx[r][x]);
+      if (i >= 0) {
+#pragma unroll
+        for (int v = 0; v < VW; v++) dn[(size_t(c) * ns + i) * VW + v] += sc * double(gx[r][x][v]);
+      }
     }
   }
 }
@@ -990,17 +1022,20 @@ void launch_fft_up(const M2LDev& m, const DevTree& t, const int* src_node, int n
   if (nslots <= 0) return;
   if (fft_node_ok(m)) {
     const size_t lds = fft_node_lds(m, (m.mixed && m.mixed_fft) ? 4 : 8);
+    auto go = [&](auto kern) {
+      allow_lds(kern, lds);
+      hipLaunchKernelGGL(kern, dim3(nslots), dim3(256), lds, s, m, t, src_node, nslots);
+    };
     node_dispatch(m.p, [&](auto P) {
       constexpr int PP = decltype(P)::value;
-      if (m.mixed && m.mixed_fft) {
-        allow_lds(k_fft_up_mfma<float, float, PP>, lds);
-        hipLaunchKernelGGL((k_fft_up_mfma<float, float, PP>), dim3(nslots), dim3(256), lds, s, m, t, src_node, nslots);
-      } else if (m.mixed) {
-        allow_lds(k_fft_up_mfma<float, double, PP>, lds);
-        hipLaunchKernelGGL((k_fft_up_mfma<float, double, PP>), dim3(nslots), dim3(256), lds, s, m, t, src_node, nslots);
+      if (m.cplx) {
+        if (m.mixed && m.mixed_fft) go(k_fft_up_mfma<float, float, PP, true>);
+        else if (m.mixed) go(k_fft_up_mfma<float, double, PP, true>);
+        else go(k_fft_up_mfma<double, double, PP, true>);
       } else {
-        allow_lds(k_fft_up_mfma<double, double, PP>, lds);
-        hipLaunchKernelGGL((k_fft_up_mfma<double, double, PP>), dim3(nslots), dim3(256), lds, s, m, t, src_node, nslots);
+        if (m.mixed && m.mixed_fft) go(k_fft_up_mfma<float, float, PP, false>);
+        else if (m.mixed) go(k_fft_up_mfma<float, double, PP, false>);
+        else go(k_fft_up_mfma<double, double, PP, false>);
       }
     });
     return;
@@ -1023,20 +1058,20 @@ void launch_ifft_dn(const M2LDev& m, const DevTree& t, const int* trg_node, cons
   if (nslots <= 0) return;
   if (fft_node_ok(m)) {
     const size_t lds = fft_node_lds(m, (m.mixed && m.mixed_fft) ? 4 : 8);
+    auto go = [&](auto kern) {
+      allow_lds(kern, lds);
+      hipLaunchKernelGGL(kern, dim3(nslots), dim3(256), lds, s, m, t, trg_node, scale, nslots);
+    };
     node_dispatch(m.p, [&](auto P) {
       constexpr int PP = decltype(P)::value;
-      if (m.mixed && m.mixed_fft) {
-        allow_lds(k_ifft_dn_mfma<float, float, PP>, lds);
-        hipLaunchKernelGGL((k_ifft_dn_mfma<float, float, PP>), dim3(nslots), dim3(256), lds, s, m, t, trg_node, scale,
-                           nslots);
-      } else if (m.mixed) {
-        allow_lds(k_ifft_dn_mfma<float, double, PP>, lds);
-        hipLaunchKernelGGL((k_ifft_dn_mfma<float, double, PP>), dim3(nslots), dim3(256), lds, s, m, t, trg_node, scale,
-                           nslots);
+      if (m.cplx) {
+        if (m.mixed && m.mixed_fft) go(k_ifft_dn_mfma<float, float, PP, true>);
+        else if (m.mixed) go(k_ifft_dn_mfma<float, double, PP, true>);
+        else go(k_ifft_dn_mfma<double, double, PP, true>);
       } else {
-        allow_lds(k_ifft_dn_mfma<double, double, PP>, lds);
-        hipLaunchKernelGGL((k_ifft_dn_mfma<double, double, PP>), dim3(nslots), dim3(256), lds, s, m, t, trg_node,
-                           scale, nslots);
+        if (m.mixed && m.mixed_fft) go(k_ifft_dn_mfma<float, float, PP, false>);
+        else if (m.mixed) go(k_ifft_dn_mfma<float, double, PP, false>);
+        else go(k_ifft_dn_mfma<double, double, PP, false>);
       }
     });
     return;

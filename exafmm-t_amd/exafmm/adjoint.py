@@ -16,16 +16,30 @@ sources and targets, with zero charges on the points that do not act in a pass (
 the source part, zero on the target part; adjoint: zero on the source part, w on the target part).
 Both passes run the GPU evaluator; nothing here computes on the CPU but the host setup.
 """
+import os
+import tempfile
+
 import numpy as np
 
 import exafmm
 
 KERNELS = ("laplace", "helmholtz", "modified_helmholtz")
+_CACHE = None
+
+
+def _default_cache(kernel, p, wavek):
+    """Operator cache file in a private temporary directory (the kernels' own default is a file in the
+    working directory)."""
+    global _CACHE
+    if _CACHE is None:
+        _CACHE = tempfile.mkdtemp(prefix="exafmm_adjoint_")
+    return os.path.join(_CACHE, "%s_p%d_k%s.dat" % (kernel, p, str(wavek).replace(" ", "")))
 
 
 def _fmm(kernel, p, ncrit, wavek, precision, filename):
     if kernel not in KERNELS:
         raise ValueError("adjoint: kernel must be one of %s, got %r" % (KERNELS, kernel))
+    filename = filename or _default_cache(kernel, p, wavek)
     mod = getattr(exafmm, kernel)
     if kernel == "laplace":
         fmm = mod.LaplaceFmm(p, ncrit, filename)
@@ -45,8 +59,7 @@ class AdjointFmm:
     object: the tree depends on them).  `forward(q)` -> (ntrg, 4) potential and gradient at the targets;
     `backward(w)` -> (dL/dq (nsrc,), dL/dy (nsrc, 3)) for L = sum_i w_i phi_i.  Charges are float64 for
     the real kernels and complex128 for Helmholtz.  `filename` caches the operators in the reference's
-    .dat layout (keep it empty for the modified Helmholtz kernel when positions change between objects:
-    its operators depend on the box size, which the cache does not record)."""
+    .dat layout (default: a file in a private temporary directory)."""
 
     def __init__(self, kernel, src, trg, p=8, ncrit=64, wavek=0.0, precision=exafmm.FP64, filename=""):
         self.kernel = kernel

@@ -18,7 +18,9 @@ import exafmm
 import oracle as O
 from exafmm import abi
 from exafmm.batch import LaplaceBatch, merge_forest, R_BOX, X_BOX
-from fmm_cases import points, rel_err
+import os
+
+from fmm_cases import CACHE, points, rel_err
 
 L = exafmm.laplace
 
@@ -33,7 +35,7 @@ def plummer_problems(sizes, seed):
 
 
 def own_tree(x, q, p, ncrit, precision=abi.FP64):
-    fmm = L.LaplaceFmm(p, ncrit)
+    fmm = L.LaplaceFmm(p, ncrit, os.path.join(CACHE, "batch_own_p%d_%d.dat" % (p, precision)))
     fmm.precision = precision
     tree = L.setup(L.init_sources(x, q), L.init_targets(x), fmm)
     return fmm, tree, fmm.flat_tree(tree), fmm.operators(), fmm.sorted_charges(tree)
@@ -93,7 +95,7 @@ def test_oracle_on_forest_equals_each_tree():
         flats.append(flat)
         qs.append(qsi)
     F, scale, soff, toff = merge_forest(flats)
-    opf = L.LaplaceFmm(p, ncrit)
+    opf = L.LaplaceFmm(p, ncrit, os.path.join(CACHE, "batch_forest_p%d.dat" % p))
     opf.set_bounds(list(X_BOX), R_BOX)
     opf.precompute()
     orc = O.Oracle(O.LAPLACE, p, F, opf.operators())
