@@ -35,9 +35,19 @@ __device__ __forceinline__ double rsqrt_sep(double r2) {
 }
 __device__ __forceinline__ float rsqrt_sep(float r2) { return __builtin_amdgcn_rsqf(r2); }
 __device__ __forceinline__ double dexp(double x) { return exp(x); }
-__device__ __forceinline__ float dexp(float x) { return expf(x); }
 __device__ __forceinline__ void dsincos(double x, double* s, double* c) { sincos(x, s, c); }
-__device__ __forceinline__ void dsincos(float x, float* s, float* c) { sincosf(x, s, c); }
+// FP32 (mixed / FLOAT near field and surface evaluations): exp through the
+// hardware 2^x, sine and cosine on the hardware units, whose argument is in
+// revolutions, after reducing x / (2 pi) to [-1/2, 1/2].  The reductions
+// cost a relative error of |x| 2^-24 (phase error |x| 6e-8 rad), the FP32
+// rounding of the arguments themselves.
+__device__ __forceinline__ float dexp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+__device__ __forceinline__ void dsincos(float x, float* s, float* c) {
+  const float t = x * 0.159154943091895336f;
+  const float f = t - __builtin_rintf(t);
+  *s = __builtin_amdgcn_sinf(f);
+  *c = __builtin_amdgcn_cosf(f);
+}
 
 // Compensated (Kahan) accumulation of a partial sum into (s, c).
 template <typename R>

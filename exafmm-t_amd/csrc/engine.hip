@@ -139,6 +139,10 @@ struct xfmm_context {
   DevBuf<char> fft_in, fft_out;
   // buffers
   DevBuf<double> up, dn, trg_value, check, tmp, gemm_scratch, near;
+  // near-coincident P2P pairs ([4] ints each) corrected in FP64 after an
+  // FP32 near field
+  DevBuf<int> close_pairs, close_off;
+  int nclose = 0;  // targets with near-coincident pairs
   DevBuf<double> src_rel, src_pack, surf_tab;
   DevBuf<float> surf_tab32;  // P2P sources relative to their leaf centre; packed (x, y, z, q)
 
@@ -191,6 +195,9 @@ struct xfmm_context {
     t.dn = dn.ptr;
     t.trg_value = trg_value.ptr;
     t.p2p_split = tune_split;
+    t.close = close_pairs.ptr;
+    t.close_off = close_off.ptr;
+    t.nclose = nclose;
     return t;
   }
   M2LDev m2l_dev() const {
@@ -406,6 +413,50 @@ void op_M2P(xfmm_context* c) {
   launch_eval_targets(c->kind, c->f32_far, MODE_M2P, t, c->wavek(), c->w_m2p.ptr, c->n_m2p, nullptr, 0, c->stream);
 }
 
+// The near-coincident pairs of the P2P lists (FP32 near field only), found
+// once per tree: a first pass with room for ntrgs / 8 pairs, a second with
+// the exact count when that was too small; then sorted by (target, source)
+// and grouped per target on the host, so that the correction is applied in a
+// fixed order.
+void find_close_pairs(xfmm_context* c) {
+  c->close_pairs.release();
+  c->close_off.release();
+  c->nclose = 0;
+  if (!c->f32_p2p || c->n_p2p <= 0) return;
+  DevBuf<int> count, found;
+  count.alloc(1);
+  int cap = std::max(1024, c->ntrgs / 8), n = 0;
+  for (int pass = 0; pass < 2; pass++) {
+    found.alloc(size_t(cap) * 4);
+    HIPCHK(hipMemsetAsync(count.ptr, 0, sizeof(int), c->stream));
+    launch_close_find(c->dev_tree(), c->w_p2p.ptr, c->n_p2p, found.ptr, cap, count.ptr, c->stream);
+    HIPCHK(hipMemcpyAsync(&n, count.ptr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (n <= cap) break;
+    cap = n;
+  }
+  n = std::min(n, cap);
+  if (n == 0) return;
+  std::vector<int> h(size_t(n) * 4);
+  HIPCHK(hipMemcpy(h.data(), found.ptr, h.size() * sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<int> order(n);
+  for (int i = 0; i < n; i++) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int a, int b) {
+    return h[4 * size_t(a)] != h[4 * size_t(b)] ? h[4 * size_t(a)] < h[4 * size_t(b)]
+                                                : h[4 * size_t(a) + 1] < h[4 * size_t(b) + 1];
+  });
+  std::vector<int> sorted(size_t(n) * 4), off;
+  for (int i = 0; i < n; i++) {
+    for (int d = 0; d < 4; d++) sorted[4 * size_t(i) + d] = h[4 * size_t(order[i]) + d];
+    if (i == 0 || sorted[4 * size_t(i)] != sorted[4 * size_t(i - 1)]) off.push_back(i);
+  }
+  off.push_back(n);
+  c->close_pairs.upload(sorted, c->stream);
+  c->close_off.upload(off, c->stream);
+  c->nclose = int(off.size()) - 1;
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
 void op_P2P(xfmm_context* c) {
   DevTree t = c->dev_tree();
   launch_eval_targets(c->kind, c->f32_p2p, MODE_P2P, t, c->wavek(), c->w_p2p.ptr, c->n_p2p, c->t_p2p.ptr,
@@ -572,12 +623,16 @@ int xfmm_create(int kernel, int p, double wavek_re, double wavek_im, int precisi
     std::vector<double> u = unit_surface(p);
     c->unit_surf.upload(u, c->stream);
     // the maps index the LDS cube of the transform kernels: must be a
-    // one-to-one map into [0, p^3)
+    // one-to-one map into [0, p^3), onto the cube's surface points (the
+    // forward transform stores only those)
     for (int up = 0; up < 2; up++) {
       std::vector<int> m = local_map(p, up == 1);
       std::vector<char> seen(size_t(p) * p * p, 0);
       for (int v : m) {
         if (v < 0 || v >= p * p * p || seen[v]) throw ArgError{"xfmm_create: surface-to-grid map is not one-to-one"};
+        const int x = v % p, y = (v / p) % p, z = v / (p * p);
+        const bool face = x == 0 || x == p - 1 || y == 0 || y == p - 1 || z == 0 || z == p - 1;
+        if (up && !face) throw ArgError{"xfmm_create: upward surface point inside the grid cube"};
         seen[v] = 1;
       }
       (up ? c->map_up : c->map_dn).upload(m, c->stream);
@@ -954,6 +1009,7 @@ int xfmm_set_tree(xfmm_context* c, const xfmm_tree* t) {
     c->nl_level_off[D + 1] = int(nr.size());
     c->nl_rows.upload(nr, st);
     c->nl_child_rows.upload(ncr, st);
+    find_close_pairs(c);
 
     // M2L: source / target slots and 64-target blocks per level
     std::vector<int> src_slot(nn, -1), trg_slot(nn, -1), srcs, trgs;

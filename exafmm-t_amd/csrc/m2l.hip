@@ -210,19 +210,18 @@ __global__ __launch_bounds__(256) void k_ifft_dn(M2LDev m, DevTree t, const int*
 // its p (or 2p) inputs once and produces all outputs of the line.
 
 // LDS of the per-node MFMA transforms (rb = bytes of the compute type):
-// forward: twiddles, the 8 children's [0,p)^3 input grids (complex for
-// Helmholtz), y-pass input [2p][8p] and z-pass input [2p][16p]; inverse:
-// twiddles, y-pass input [4p][8p] and the y-pass output [p][p][8] complex.
+// forward: twiddles, the 8 children's densities on the surface of the
+// [0,p)^3 grid (complex for Helmholtz), y-pass input [2p][8p] and z-pass
+// input [2p][16p]; inverse: twiddles, y-pass input [4p][8p] and the y-pass
+// output [p][p][8] complex.
 static size_t fft_node_lds(const M2LDev& m, size_t rb) {
   const size_t p = m.p, n1 = m.n1, vw = m.cplx ? 2 : 1;
-  const size_t fwd = n1 * 2 * rb + NCHILD * p * p * p * vw * rb + 2 * p * 8 * p * rb + 2 * p * 16 * p * rb;
+  const size_t fwd = n1 * 2 * rb + NCHILD * size_t(m.nsurf) * vw * rb + 2 * p * 8 * p * rb + 2 * p * 16 * p * rb;
   const size_t inv = n1 * 2 * rb + 4 * p * 8 * p * rb + p * p * NCHILD * 2 * rb;
   return fwd > inv ? fwd : inv;
 }
 
-// per-node transforms: p in [3, 12], LDS within the CU's 160 KiB (the
-// complex FP64 forward grid at p = 10 is not: Helmholtz in FP64 keeps the
-// per-child transforms there)
+// per-node transforms: p in [3, 12], LDS within the CU's 160 KiB
 static bool fft_node_ok(const M2LDev& m) {
   return m.p >= 3 && m.p <= 12 && fft_node_lds(m, (m.mixed && m.mixed_fft) ? 4 : 8) <= 160 * 1024;
 }
@@ -694,6 +693,26 @@ __device__ __forceinline__ double dft_entry(int ph, int n1, int sign, int ro, in
   return ro == 0 ? (ri == 0 ? cs : -sn) : (ri == 0 ? sn : cs);
 }
 
+// Compact LDS storage of a child's upward equivalent density on the
+// [0,p)^3 grid: the density lives on the cube's surface points only, stored
+// line by line along x, lines (z, y) in z-major order; a line on a face of
+// the cube (y or z in {0, p-1}) holds all p points, an interior line only
+// x = 0 and x = p - 1 -- 6 (p-1)^2 + 2 = nsurf values in all.
+template <int P>
+__device__ __forceinline__ int surf_line_off(int z, int y) {
+  constexpr int SL = 4 * P - 4;  // values of an interior z slab
+  if (z == 0) return y * P;
+  if (z == P - 1) return P * P + (P - 2) * SL + y * P;
+  const int b = P * P + (z - 1) * SL;
+  return y == 0 ? b : (y == P - 1 ? b + P + 2 * (P - 2) : b + P + 2 * (y - 1));
+}
+template <int P>
+__device__ __forceinline__ int surf_compact(int v) {  // v = x + P (y + P z), a surface point
+  const int x = v % P, y = (v / P) % P, z = v / (P * P);
+  const bool full = z == 0 || z == P - 1 || y == 0 || y == P - 1;
+  return surf_line_off<P>(z, y) + (full ? x : (x == 0 ? 0 : 1));
+}
+
 // fft_up_equiv for one node: out[(kx n1^2 + ky + n1 kz) * 8 + c] = sum over
 // the child's surface points (x,y,z) of up[c][i] exp(-2 pi i (kx x + ky y +
 // kz z) / n1), kx over the r2c half axis [0, p] for the real kernels
@@ -710,31 +729,27 @@ __global__ __launch_bounds__(256) void k_fft_up_mfma(M2LDev m, DevTree t, const 
   constexpr int MT = (2 * n1 + 15) / 16;             // rows: (ky or kz, re/im)
   constexpr int NCY = 8 * P, NTY = (NCY + 15) / 16;  // y pass columns (z, c)
   constexpr int NCZ = 8 * n1, NTZ = NCZ / 16;        // z pass columns (ky, c)
+  constexpr int NS = 6 * (P - 1) * (P - 1) + 2;     // surface points per child
+  constexpr int NFL = 4 * P - 4, NSL = (P - 2) * (P - 2);  // face lines, interior lines
   extern __shared__ char smem_raw[];
   const int slot = blockIdx.x;
   if (slot >= nslots) return;
-  const int ns = m.nsurf;
-  constexpr int P3 = P * P * P;
   C2<R>* w = reinterpret_cast<C2<R>*>(smem_raw);  // exp(-2 pi i m / n1)
-  R* gx = reinterpret_cast<R*>(w + n1);           // [c][z][y][x] input grid (x VW reals)
-  R* Ay = gx + NCHILD * P3 * VW;                  // [(y, re/im)][(z, c)]
+  R* gx = reinterpret_cast<R*>(w + n1);           // [c][compact surface index] (x VW reals)
+  R* Ay = gx + NCHILD * NS * VW;                  // [(y, re/im)][(z, c)]
   R* Bz = Ay + KK * NCY;                          // [(z, re/im)][(ky, c)]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int j = lane & 15, g = lane >> 4;
   make_twiddles(w, n1, -1);
-  for (int e = tid; e < NCHILD * P3 * VW; e += 256) gx[e] = R(0);
-  __syncthreads();
   const int node = src_node[slot];
-  const double* up = t.up + size_t(t.child0[node]) * ns * VW;
-  for (int e = tid; e < NCHILD * ns; e += 256) {
-    const int c = e / ns, i = e - c * ns;
-    if constexpr (CPLX) {
-      R* gp = gx + 2 * (c * P3 + m.map_up[i]);
-      gp[0] = R(up[2 * e]);
-      gp[1] = R(up[2 * e + 1]);
-    } else {
-      gx[c * P3 + m.map_up[i]] = R(up[e]);
-    }
+  const double* up = t.up + size_t(t.child0[node]) * NS * VW;
+  // every compact slot is one surface point (map_up is one-to-one onto the
+  // cube's surface, checked when the context is created)
+  for (int e = tid; e < NCHILD * NS; e += 256) {
+    const int c = e / NS, i = e - c * NS;
+    R* gp = gx + (c * NS + surf_compact<P>(m.map_up[i])) * VW;
+    gp[0] = R(up[size_t(e) * VW]);
+    if constexpr (CPLX) gp[1] = R(up[2 * size_t(e) + 1]);
   }
   R wf[MT][KS];
 #pragma unroll
@@ -748,23 +763,46 @@ __global__ __launch_bounds__(256) void k_fft_up_mfma(M2LDev m, DevTree t, const 
   C2<RS>* out = reinterpret_cast<C2<RS>*>(m.fft_in) + size_t(slot) * m.nfreq * NCHILD;
   float mx = 0.f;
   for (int kx = 0; kx < NKX; kx++) {
-    // x: Ay[(y, re/im)][(z, c)] = sum_x gx[c][z][y][x] exp(-2 pi i kx x / n1)
+    // x: Ay[(y, re/im)][(z, c)] = sum_x gx[c][z][y][x] exp(-2 pi i kx x / n1),
+    // the face lines (all p points) first, then the interior lines (x = 0
+    // and x = p - 1 only), so that no wave mixes the two
     C2<R> tw[P];
 #pragma unroll
     for (int x = 0; x < P; x++) tw[x] = w[(kx * x) % n1];
-    for (int e = tid; e < NCHILD * P * P; e += 256) {
-      const int c = e & 7, line = e >> 3, y = line / P, z = line % P;
-      const R* gl = gx + (c * P3 + (z * P + y) * P) * VW;
+    auto line_sum = [&](const R* gl, int x, C2<R>& acc, const C2<R>& tx) {
+      if constexpr (CPLX) {
+        cfma(acc, C2<R>{gl[2 * x], gl[2 * x + 1]}, tx);
+      } else {
+        acc.x = fma(gl[x], tx.x, acc.x);
+        acc.y = fma(gl[x], tx.y, acc.y);
+      }
+    };
+    for (int e = tid; e < NCHILD * NFL; e += 256) {
+      const int c = e & 7, f = e >> 3;
+      int z, y;
+      if (f < P) {
+        z = 0;
+        y = f;
+      } else if (f < 2 * P) {
+        z = P - 1;
+        y = f - P;
+      } else {
+        z = 1 + ((f - 2 * P) >> 1);
+        y = ((f - 2 * P) & 1) ? P - 1 : 0;
+      }
+      const R* gl = gx + (c * NS + surf_line_off<P>(z, y)) * VW;
       C2<R> acc{0, 0};
 #pragma unroll
-      for (int x = 0; x < P; x++) {
-        if constexpr (CPLX) {
-          cfma(acc, C2<R>{gl[2 * x], gl[2 * x + 1]}, tw[x]);
-        } else {
-          acc.x = fma(gl[x], tw[x].x, acc.x);
-          acc.y = fma(gl[x], tw[x].y, acc.y);
-        }
-      }
+      for (int x = 0; x < P; x++) line_sum(gl, x, acc, tw[x]);
+      Ay[(2 * y) * NCY + z * 8 + c] = acc.x;
+      Ay[(2 * y + 1) * NCY + z * 8 + c] = acc.y;
+    }
+    for (int e = tid; e < NCHILD * NSL; e += 256) {
+      const int c = e & 7, sl = e >> 3, z = 1 + sl / (P - 2), y = 1 + sl % (P - 2);
+      const R* gl = gx + (c * NS + surf_line_off<P>(z, y)) * VW;
+      C2<R> acc{0, 0};
+      line_sum(gl, 0, acc, tw[0]);
+      line_sum(gl, 1, acc, tw[P - 1]);
       Ay[(2 * y) * NCY + z * 8 + c] = acc.x;
       Ay[(2 * y + 1) * NCY + z * 8 + c] = acc.y;
     }

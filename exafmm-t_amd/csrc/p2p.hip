@@ -25,23 +25,24 @@ constexpr bool kcplx() { return KIND == HELMHOLTZ; }
 
 // 1/r of a pair.  RSQ_LIB: library rsqrt, r = 0 gives 0 (the direct sum used as
 // accuracy check).  RSQ_MASK: hardware estimate refined to full precision
-// (rsqrt_sep), r = 0 gives 0 (a leaf with itself, where a target may coincide
-// with a source).  RSQ_SEP: rsqrt_sep without the test (pairs that cannot
-// coincide: distinct leaves own distinct points, and surfaces never touch
-// particles).
+// (rsqrt_sep), r^2 <= rmin2 gives 0 (a leaf with itself, where a target may
+// coincide with a source; the FP32 near field also leaves out the
+// near-coincident pairs that k_close_fix adds in FP64).  RSQ_SEP: rsqrt_sep
+// without the test (pairs that cannot coincide: distinct leaves own distinct
+// points, and surfaces never touch particles).
 enum RsqMode { RSQ_LIB = 0, RSQ_MASK = 1, RSQ_SEP = 2 };
 template <int RM, typename R>
-__device__ __forceinline__ R inv_r(R r2) {
+__device__ __forceinline__ R inv_r(R r2, R rmin2) {
   if constexpr (RM == RSQ_LIB) return rsqrt_pos(r2);
-  else if constexpr (RM == RSQ_MASK) return r2 > R(0) ? rsqrt_sep(r2) : R(0);
+  else if constexpr (RM == RSQ_MASK) return r2 > rmin2 ? rsqrt_sep(r2) : R(0);
   else return rsqrt_sep(r2);
 }
 
 // potential + gradient of one pair, accumulated into a[] (4 or 8 values)
 template <int KIND, typename R, int RM>
-__device__ __forceinline__ void pair_grad(R dx, R dy, R dz, R qr, R qi, R kr, R ki, R* a) {
+__device__ __forceinline__ void pair_grad(R dx, R dy, R dz, R qr, R qi, R kr, R ki, R* a, R rmin2 = R(0)) {
   const R r2 = dx * dx + dy * dy + dz * dz;
-  const R invr = inv_r<RM>(r2);
+  const R invr = inv_r<RM>(r2, rmin2);
   if constexpr (KIND == LAPLACE) {
     const R qinv = qr * invr;
     a[0] += qinv;
@@ -84,7 +85,7 @@ __device__ __forceinline__ void pair_grad(R dx, R dy, R dz, R qr, R qi, R kr, R 
 template <int KIND, typename R, int RM>
 __device__ __forceinline__ void pair_pot(R dx, R dy, R dz, R qr, R qi, R kr, R ki, R* a) {
   const R r2 = dx * dx + dy * dy + dz * dz;
-  const R invr = inv_r<RM>(r2);
+  const R invr = inv_r<RM>(r2, R(0));
   if constexpr (KIND == LAPLACE) {
     a[0] += qr * invr;
   } else if constexpr (KIND == YUKAWA) {
@@ -117,7 +118,7 @@ constexpr int TILE = 64;
 // four pairs per loop trip so that the LDS reads of a trip are issued together.
 template <int KIND, typename R, int RM, bool GRAD>
 __device__ __forceinline__ void tile_sum(int m, const R* sx, const R* sy, const R* sz, const R* sq, const R* sqi, R tx,
-                                         R ty, R tz, R kr, R ki, R* part) {
+                                         R ty, R tz, R kr, R ki, R* part, R rmin2 = R(0)) {
   constexpr bool C = kcplx<KIND>();
   int j = 0;
   for (; j + 4 <= m; j += 4) {
@@ -125,7 +126,7 @@ __device__ __forceinline__ void tile_sum(int m, const R* sx, const R* sy, const 
     for (int u = 0; u < 4; u++) {
       if constexpr (GRAD)
         pair_grad<KIND, R, RM>(sx[j + u] - tx, sy[j + u] - ty, sz[j + u] - tz, sq[j + u], C ? sqi[j + u] : R(0), kr, ki,
-                               part);
+                               part, rmin2);
       else
         pair_pot<KIND, R, RM>(sx[j + u] - tx, sy[j + u] - ty, sz[j + u] - tz, sq[j + u], C ? sqi[j + u] : R(0), kr, ki,
                               part);
@@ -133,10 +134,24 @@ __device__ __forceinline__ void tile_sum(int m, const R* sx, const R* sy, const 
   }
   for (; j < m; j++) {
     if constexpr (GRAD)
-      pair_grad<KIND, R, RM>(sx[j] - tx, sy[j] - ty, sz[j] - tz, sq[j], C ? sqi[j] : R(0), kr, ki, part);
+      pair_grad<KIND, R, RM>(sx[j] - tx, sy[j] - ty, sz[j] - tz, sq[j], C ? sqi[j] : R(0), kr, ki, part, rmin2);
     else
       pair_pot<KIND, R, RM>(sx[j] - tx, sy[j] - ty, sz[j] - tz, sq[j], C ? sqi[j] : R(0), kr, ki, part);
   }
+}
+
+// Squared distance below which the FP32 near field of a leaf with itself
+// leaves a pair to k_close_fix: (side / 16)^2 of the target leaf when the
+// near-coincident pairs are listed (t.close), else 0 (only r = 0 masked).
+template <typename R>
+__device__ __forceinline__ R close_rmin2(const DevTree& t, int leaf) {
+  if constexpr (sizeof(R) == 4) {
+    if (t.close) {
+      const double side = 2.0 * t.r0 * ldexp(1.0, -t.level[leaf]);
+      return R(side * side * (1.0 / 256.0));
+    }
+  }
+  return R(0);
 }
 
 // ------------------------------------------------------------------ P2P
@@ -218,6 +233,7 @@ __global__ __launch_bounds__(64) void k_p2p_smem(DevTree t, Wavek wk, const int*
   const int k = split ? TILE / nt : 1;
   const int ti = split ? lane % nt : lane, ph = split ? lane / nt : 0;
   const bool act = split ? ph < k : ti < nt;
+  const R rmin2 = close_rmin2<R>(t, leaf);
   R tx = 0, ty = 0, tz = 0;
   if (ti < nt) {
     const double* x = t.trg + 3 * size_t(tb + t0 + ti);
@@ -242,7 +258,7 @@ __global__ __launch_bounds__(64) void k_p2p_smem(DevTree t, Wavek wk, const int*
         if (self)
           for (int j = ph; j < ns; j += k) {
             const V v = srcv[sb + j];
-            pair_grad<KIND, R, RSQ_MASK>(v.x - ux, v.y - uy, v.z - uz, v.w, R(0), kr, ki, part);
+            pair_grad<KIND, R, RSQ_MASK>(v.x - ux, v.y - uy, v.z - uz, v.w, R(0), kr, ki, part, rmin2);
           }
         else
           for (int j = ph; j < ns; j += k) {
@@ -263,12 +279,12 @@ __global__ __launch_bounds__(64) void k_p2p_smem(DevTree t, Wavek wk, const int*
 #pragma unroll
           for (int u = 0; u < 4; u++) {
             const CR v = S + 4 * (j + u);
-            pair_grad<KIND, R, RSQ_MASK>(v[0] - ux, v[1] - uy, v[2] - uz, v[3], R(0), kr, ki, part);
+            pair_grad<KIND, R, RSQ_MASK>(v[0] - ux, v[1] - uy, v[2] - uz, v[3], R(0), kr, ki, part, rmin2);
           }
         }
         for (; j < m; j++) {
           const CR v = S + 4 * j;
-          pair_grad<KIND, R, RSQ_MASK>(v[0] - ux, v[1] - uy, v[2] - uz, v[3], R(0), kr, ki, part);
+          pair_grad<KIND, R, RSQ_MASK>(v[0] - ux, v[1] - uy, v[2] - uz, v[3], R(0), kr, ki, part, rmin2);
         }
       } else {
         for (; j + 4 <= m; j += 4) {
@@ -444,7 +460,8 @@ __global__ __launch_bounds__(64) void k_eval_targets(DevTree t, Wavek wk, const 
         // only a leaf with itself can pair coinciding points; equivalent
         // surfaces never touch the targets
         if (MODE == MODE_P2P && snode == leaf)
-          tile_sum<KIND, R, RSQ_MASK, true>(m, sx, sy, sz, sq, sqi, tx, ty, tz, kr, ki, part);
+          tile_sum<KIND, R, RSQ_MASK, true>(m, sx, sy, sz, sq, sqi, tx, ty, tz, kr, ki, part,
+                                            close_rmin2<R>(t, leaf));
         else
           tile_sum<KIND, R, RSQ_SEP, true>(m, sx, sy, sz, sq, sqi, tx, ty, tz, kr, ki, part);
         fold_tile<R, NA>(acc, comp, part);
@@ -584,6 +601,116 @@ __global__ void k_direct_reduce(const double* __restrict__ partial, int nsplit, 
   out[i] += (s - c) * FOUR_PI_INV;
 }
 
+// ------------------------------------------------- near-coincident pairs
+//
+// The FP32 near field forms a pair's difference vector from coordinates
+// relative to leaf centres, each rounded to FP32: an absolute error of about
+// 2^-24 of the leaf size per coordinate, so a relative error of about
+// 2^-24 L / r in 1/r (and twice that in the gradient) for a pair at
+// distance r.  Only pairs much closer than the leaf size are hurt, and they
+// are rare: they are listed once per tree (k_close_find, FP64 distances) and
+// after every FP32 near field k_close_fix replaces each one's FP32
+// contribution, recomputed from the very operands the near-field kernel used,
+// by its FP64 value.
+__global__ __launch_bounds__(64) void k_close_find(DevTree t, const int* __restrict__ work, int* __restrict__ out,
+                                                   int cap, int* __restrict__ count) {
+  const int leaf = work[blockIdx.x];
+  const int tb = t.trg_begin[leaf], nt = t.trg_count[leaf];
+  const double side = 2.0 * t.r0 * ldexp(1.0, -t.level[leaf]);
+  // (side / 16)^2 with a margin, so that every pair the FP32 near field
+  // leaves out (FP32 r^2 <= (side / 16)^2, close_rmin2) is listed
+  const double d2 = side * side * (1.002 / 256.0);
+  const int g0 = t.p2p_off[leaf], g1 = t.p2p_off[leaf + 1];
+  for (int i = threadIdx.x; i < nt; i += blockDim.x) {
+    const double* x = t.trg + 3 * size_t(tb + i);
+    const double x0 = x[0], x1 = x[1], x2 = x[2];
+    for (int g = g0; g < g1; g++) {
+      const int sn = t.p2p_src[g], sb = t.src_begin[sn], ns = t.src_count[sn];
+      for (int j = 0; j < ns; j++) {
+        const double* y = t.src + 3 * size_t(sb + j);
+        const double dx = y[0] - x0, dy = y[1] - x1, dz = y[2] - x2;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 < d2 && r2 > 0.0) {
+          const int k = atomicAdd(count, 1);
+          if (k < cap) {
+            int* o = out + 4 * size_t(k);
+            o[0] = tb + i;
+            o[1] = sb + j;
+            o[2] = leaf;
+            o[3] = sn;
+          }
+        }
+      }
+    }
+  }
+}
+
+void launch_close_find(const DevTree& t, const int* work, int nwork, int* out, int cap, int* count, hipStream_t s) {
+  if (nwork <= 0) return;
+  hipLaunchKernelGGL(k_close_find, dim3(nwork), dim3(64), 0, s, t, work, out, cap, count);
+}
+
+// One thread per target with listed pairs, its pairs in list order (so the
+// result does not depend on scheduling): trg_value += sum of (FP64 - FP32
+// contribution) / 4 pi.  The FP32 operands are formed as in k_p2p_smem
+// (PACKED, real kernels: packed source relative to its leaf centre, target
+// moved into the source leaf's frame) or k_eval_targets (both relative to
+// the target leaf centre).
+template <int KIND, bool PACKED>
+__device__ __forceinline__ void close_pair_fix(const DevTree& t, Wavek wk, const int* cp, double* fix) {
+  constexpr bool C = kcplx<KIND>();
+  constexpr int NA = C ? 8 : 4;
+  const int it = cp[0], is = cp[1], tl = cp[2], sl = cp[3];
+  const double cx = t.center[3 * tl], cy = t.center[3 * tl + 1], cz = t.center[3 * tl + 2];
+  const double* x = t.trg + 3 * size_t(it);
+  const double* y = t.src + 3 * size_t(is);
+  const float tx = float(x[0] - cx), ty = float(x[1] - cy), tz = float(x[2] - cz);
+  const float kr = float(wk.re), ki = float(wk.im);
+  const double qr = t.q[size_t(is) * t.vw], qi = C ? t.q[size_t(is) * t.vw + 1] : 0.0;
+  float a32[NA];
+  double a64[NA];
+#pragma unroll
+  for (int a = 0; a < NA; a++) {
+    a32[a] = 0.f;
+    a64[a] = 0.0;
+  }
+  float dx, dy, dz, q32;
+  if constexpr (PACKED) {
+    const float4 v = static_cast<const float4*>(t.src_pack)[is];
+    dx = v.x - (tx - float(t.center[3 * sl] - cx));
+    dy = v.y - (ty - float(t.center[3 * sl + 1] - cy));
+    dz = v.z - (tz - float(t.center[3 * sl + 2] - cz));
+    q32 = v.w;
+  } else {
+    dx = float(y[0] - cx) - tx;
+    dy = float(y[1] - cy) - ty;
+    dz = float(y[2] - cz) - tz;
+    q32 = float(qr);
+  }
+  // a leaf with itself: the near-field kernel left out the pairs with FP32
+  // r^2 <= rmin2 (their FP32 value is 0 here too)
+  if (tl == sl) pair_grad<KIND, float, RSQ_MASK>(dx, dy, dz, q32, float(qi), kr, ki, a32, close_rmin2<float>(t, tl));
+  else pair_grad<KIND, float, RSQ_SEP>(dx, dy, dz, q32, float(qi), kr, ki, a32);
+  pair_grad<KIND, double, RSQ_MASK>(y[0] - x[0], y[1] - x[1], y[2] - x[2], qr, qi, wk.re, wk.im, a64);
+#pragma unroll
+  for (int a = 0; a < NA; a++) fix[a] += a64[a] - double(a32[a]);
+}
+
+template <int KIND, bool PACKED>
+__global__ __launch_bounds__(256) void k_close_fix(DevTree t, Wavek wk) {
+  constexpr int NA = kcplx<KIND>() ? 8 : 4;
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= t.nclose) return;
+  double fix[NA];
+#pragma unroll
+  for (int a = 0; a < NA; a++) fix[a] = 0.0;
+  const int k0 = t.close_off[u], k1 = t.close_off[u + 1];
+  for (int k = k0; k < k1; k++) close_pair_fix<KIND, PACKED>(t, wk, t.close + 4 * size_t(k), fix);
+  double* o = t.trg_value + size_t(t.close[4 * size_t(k0)]) * 4 * t.vw;
+#pragma unroll
+  for (int a = 0; a < NA; a++) o[a] += fix[a] * FOUR_PI_INV;
+}
+
 // ---------------------------------------------------------------- launchers
 
 template <int KIND, typename R>
@@ -600,6 +727,14 @@ static void eval_targets_t(int mode, const DevTree& t, Wavek k, const int* work,
   else if (mode == MODE_M2P) hipLaunchKernelGGL((k_eval_targets<KIND, R, MODE_M2P>), dim3(nwork), dim3(64), 0, s, t, k, work);
   else if (real && t.surf_tab) hipLaunchKernelGGL((k_l2p_smem<KIND, R>), dim3(ntiles), dim3(64), 0, s, t, k, tiles);
   else hipLaunchKernelGGL((k_eval_targets<KIND, R, MODE_L2P>), dim3(nwork), dim3(64), 0, s, t, k, work);
+  // FP32 near field: the near-coincident pairs in FP64 (same stream, after it)
+  if constexpr (sizeof(R) == 4) {
+    if (mode == MODE_P2P && t.nclose > 0 && t.close && t.close_off) {
+      const dim3 g((t.nclose + 255) / 256);
+      if (real && t.src_pack && t.src_rel) hipLaunchKernelGGL((k_close_fix<KIND, true>), g, dim3(256), 0, s, t, k);
+      else hipLaunchKernelGGL((k_close_fix<KIND, false>), g, dim3(256), 0, s, t, k);
+    }
+  }
 }
 
 void launch_eval_targets(int kind, bool f32, int mode, const DevTree& t, Wavek k, const int* work, int nwork,

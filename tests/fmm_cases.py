@@ -31,6 +31,13 @@ def points(n, dist, rng):
         r = np.minimum(r, 10.0)
         v = rng.normal(size=(n, 3))
         return v / np.linalg.norm(v, axis=1)[:, None] * r[:, None]
+    if dist == "pairs":
+        # uniform, a tenth of the points with a partner at distance 10^U(-7, -2)
+        m = n // 10
+        x = rng.random((n - m, 3))
+        d = rng.normal(size=(m, 3))
+        d *= (10.0 ** rng.uniform(-7.0, -2.0, m) / np.linalg.norm(d, axis=1))[:, None]
+        return np.concatenate([x, np.clip(x[:m] + d, 0.0, 1.0)])
     if dist == "clusters":
         c = rng.random((4, 3))
         k = rng.integers(0, 4, n)

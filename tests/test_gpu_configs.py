@@ -172,3 +172,33 @@ def test_mixed_stage_errors(require_gpu, kernel):
             os.environ.pop("XFMM_MIXED_STAGES", None)
         else:
             os.environ["XFMM_MIXED_STAGES"] = saved
+
+
+@pytest.mark.parametrize("kernel", ["laplace", "modified_helmholtz", "helmholtz"])
+def test_near_coincident_pairs(require_gpu, kernel):
+    """FP32 near field on points with near-coincident partners (a tenth of the points at distance
+    10^U(-7, -2) of another): the pairs closer than 1/16 of the leaf side are evaluated in FP64, so the
+    FP32 near field alone (XFMM_MIXED_STAGES=p2p) stays within 1e-6 (potential) and 5e-6 (gradient) of
+    the oracle; without that, a pair at 1e-7 costs the FP32 coordinate rounding ~2^-24 of the leaf size
+    relative to 1e-7, i.e. an error of order one in its targets' values."""
+    import os
+    pr = make_problem(kernel, 30000, 6, ncrit=64, dist="pairs", seed=17, charge="normal")
+    orc = O.Oracle(pr["kid"], pr["p"], pr["flat"], pr["ops"], pr["wavek"])
+    orc.set_charges(pr["qs"])
+    ref = orc.evaluate()
+    saved = os.environ.get("XFMM_MIXED_STAGES")
+    try:
+        os.environ["XFMM_MIXED_STAGES"] = "p2p"
+        eng = _engine(pr, abi.MIXED)
+        for _ in range(2):
+            eng.upward_pass()
+            eng.downward_pass()
+            e = errs(eng.values(), ref)
+            print("%s pairs N=30000 FP32 near field vs oracle phi %.2e grad %.2e" % (kernel, e[0], e[1]), flush=True)
+            assert e[0] < 1e-6 and e[1] < 5e-6, e
+        eng.close()
+    finally:
+        if saved is None:
+            os.environ.pop("XFMM_MIXED_STAGES", None)
+        else:
+            os.environ["XFMM_MIXED_STAGES"] = saved
