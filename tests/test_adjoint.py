@@ -132,15 +132,12 @@ def test_adjoint_fmm_vs_direct(require_gpu, kernel, distinct):
 @pytest.mark.parametrize("distinct", [False, True])
 def test_adjoint_vs_oracle_yukawa_sphere(require_gpu, distinct):
     """The adjoint evaluation (w at the targets, values at the sources) at the Yukawa workload's
-    parameters against the oracle on the same tree and operators: FP64 1e-10; mixed 1e-4 for the
-    potential and 2e-4 for the gradient.  The mixed error here is that of the FP32 near field on the
-    sphere: near-coincident pairs (r ~ 1e-4 of the leaf size) resolved in FP32 coordinates relative to
-    the leaf centre (measured at this seed, sources = targets: 2.7e-5 / 9.4e-5; distinct sets below
-    1e-5; every far-field stage alone stays below 1e-7 on the sphere)."""
+    parameters against the oracle on the same tree and operators: FP64 1e-10, mixed 1e-5 (measured
+    7.9e-8 / 9.1e-10 with sources = targets, 9.1e-8 / 6.3e-9 for distinct sets)."""
     from exafmm.adjoint import AdjointFmm
     n = 40000 if not distinct else 20000
     y, x, q, w = _case("modified_helmholtz", n, seed=11, distinct=distinct)
-    for prec, tol, gtol in ((abi.FP64, 1e-10, 1e-10), (abi.MIXED, 1e-4, 2e-4)):
+    for prec, tol, gtol in ((abi.FP64, 1e-10, 1e-10), (abi.MIXED, 1e-5, 1e-5)):
         a = AdjointFmm("modified_helmholtz", y, x, p=8, ncrit=64, wavek=5.0, precision=prec)
         flat = a.flat_tree()
         m2p, p2l = int(flat["m2p_offset"][-1]), int(flat["p2l_offset"][-1])

@@ -12,11 +12,11 @@ distribution, expansion order, ncrit and wave number of each workload.
 * Modified Helmholtz k = 5, unit sphere (normalised Gaussian samples), charges
   N(0,1), p = 8, ncrit = 64, N = 50k, an adaptive tree whose M2P and P2L lists
   are non-empty (asserted): FP64 per operator and full passes within 1e-10;
-  mixed path within 1e-5 (gradient 5e-5: near-coincident pairs on the sphere,
-  see the test).
+  mixed path within 1e-5 (the near-coincident pairs of the sphere evaluated
+  in FP64).
 * Helmholtz k = 20 (real), uniform cube, complex charges U[-1,1) + i U[-1,1),
   p = 10, ncrit = 128, N = 30k: FP64 full passes within 1e-10; the complex64
-  mixed path within 1e-5 (gradient 3e-5: the FP32 near field, see the test).
+  mixed path within 1e-5.
 """
 import numpy as np
 import pytest
@@ -24,7 +24,7 @@ import pytest
 import oracle as O
 from exafmm import abi
 from fmm_cases import make_problem, rel_err, truncated_ops
-from test_gpu_parity import FP64_TOL, MIXED_TOL, _engine, check_stages_fp64, grad_tol_fp32
+from test_gpu_parity import FP64_TOL, MIXED_TOL, _engine, check_stages_fp64
 
 pytestmark = pytest.mark.gpu
 
@@ -48,10 +48,10 @@ def errs(v, r):
     return rel_err(v[:, 0], r[:, 0]), rel_err(v[:, 1:], r[:, 1:])
 
 
-def check_truncated_paths(pr, precisions, idx, d, ref64, grad_tol=MIXED_TOL):
+def check_truncated_paths(pr, precisions, idx, d, ref64):
     """Mixed / FLOAT engine against the oracle on the truncated operators (1e-5
-    for the potential, grad_tol for the gradient) and against the FP64 direct
-    sum (1e-5, or twice the FP64 FMM's own error where that is larger)."""
+    for the potential and the gradient) and against the FP64 direct sum (1e-5,
+    or twice the FP64 FMM's own error where that is larger)."""
     ops_t = truncated_ops(pr)
     orc = O.Oracle(pr["kid"], pr["p"], pr["flat"], ops_t, pr["wavek"])
     orc.set_charges(pr["qs"])
@@ -66,7 +66,7 @@ def check_truncated_paths(pr, precisions, idx, d, ref64, grad_tol=MIXED_TOL):
         eo, ed = errs(v, ref_t), errs(v[idx], d)
         print("precision %d vs oracle phi %.2e grad %.2e; vs direct phi %.2e grad %.2e (fp64 fmm %.2e %.2e)"
               % (prec, eo[0], eo[1], ed[0], ed[1], e64[0], e64[1]), flush=True)
-        assert eo[0] < MIXED_TOL and eo[1] < grad_tol, (prec, eo)
+        assert eo[0] < MIXED_TOL and eo[1] < MIXED_TOL, (prec, eo)
         assert ed[0] < max(2 * e64[0], MIXED_TOL) and ed[1] < max(2 * e64[1], MIXED_TOL), (prec, ed, e64)
 
 
@@ -108,10 +108,10 @@ def test_yukawa_sphere_adaptive(require_gpu):
     assert ls["m2p"] > 0 and ls["p2l"] > 0, ls
     ref = check_stages_fp64(pr)
     idx, d = direct_sample(pr, 1000, 5)
-    # near-coincident pairs on the sphere: see grad_tol_fp32 (measured: mixed
-    # vs oracle gradient 1.4e-5, vs the FP64 direct sum on 1,000 sampled
-    # targets 2.8e-6)
-    check_truncated_paths(pr, (abi.MIXED,), idx, d, ref, grad_tol=grad_tol_fp32("sphere"))
+    # the sphere's near-coincident pairs (nearest-neighbour distance down to
+    # 1e-4 of the median) are evaluated in FP64 by the FP32 near field
+    # (measured: mixed vs oracle phi 9.0e-8, gradient 6.5e-9)
+    check_truncated_paths(pr, (abi.MIXED,), idx, d, ref)
 
 
 def test_helmholtz_k20_parameters(require_gpu):
@@ -129,23 +129,20 @@ def test_helmholtz_k20_parameters(require_gpu):
         assert e[0] < FP64_TOL and e[1] < FP64_TOL
     eng.close()
     idx, d = direct_sample(pr, 1000, 9)
-    # the gradient error of the mixed path is that of its FP32 near field (per
-    # stage against the oracle: FP32 P2P alone 1.4e-5, FP32 M2L 5e-8, FP32
-    # check potentials / L2P / translations <= 1e-8): close pairs of the
-    # uniform points (r ~ 1e-3 of the leaf size) resolved to FP32 box-relative
-    # coordinates; against the FP64 direct sum on 1,000 targets it stays < 1e-5
-    check_truncated_paths(pr, (abi.MIXED,), idx, d, ref, grad_tol=3e-5)
+    # measured: mixed vs oracle phi 9.8e-7, gradient 6.0e-8 (the FP32 M2L
+    # dominates the potential error; the near field's close pairs are FP64)
+    check_truncated_paths(pr, (abi.MIXED,), idx, d, ref)
 
 
 @pytest.mark.parametrize("kernel", ["laplace", "helmholtz"])
 def test_mixed_stage_errors(require_gpu, kernel):
     """Each FP32 stage of the mixed path on its own (XFMM_MIXED_STAGES) against
     the oracle on the same truncated operators, at the headline parameters
-    (Laplace p = 10, ncrit = 128) and the Helmholtz k = 20 ones: the far-field
-    stages each stay far below 1e-5 (M2L transforms + split-FP16 Hadamard
-    3e-6, measured 1.0e-6 for Helmholtz; check potentials, L2P / M2P
-    evaluation, M2M / L2L 1e-6); the FP32 near field is the one stage whose
-    gradient error reaches 1e-5."""
+    (Laplace p = 10, ncrit = 128) and the Helmholtz k = 20 ones: every stage
+    stays far below 1e-5 (M2L transforms + split-FP16 Hadamard 3e-6,
+    measured 1.0e-6 for Helmholtz; check potentials, L2P / M2P evaluation,
+    M2M / L2L and the near field with its near-coincident pairs in FP64
+    1e-6, measured <= 1.7e-7)."""
     import os
     if kernel == "laplace":
         pr = make_problem("laplace", 60000, 10, ncrit=128, dist="cube", seed=40, charge="u11")
@@ -158,7 +155,7 @@ def test_mixed_stage_errors(require_gpu, kernel):
     saved = os.environ.get("XFMM_MIXED_STAGES")
     try:
         for stages, tol in (("m2l,had16,fft", 3e-6), ("surf", 1e-6), ("far", 1e-6), ("trans", 1e-6),
-                            ("p2p", 5e-5)):
+                            ("p2p", 1e-6)):
             os.environ["XFMM_MIXED_STAGES"] = stages
             eng = _engine(pr, abi.MIXED, ops_t)
             eng.upward_pass()
