@@ -151,6 +151,7 @@ struct xfmm_context {
   DevBuf<double> vscale;
   // tuning (xfmm_set_tuning / xfmm_autotune)
   int tune_kch = HAD_KCH;  // Hadamard frequencies per workgroup
+  int tune_groups = 2;     // split-FP16 Hadamard: 64-target blocks per workgroup
   int tune_split = 32;     // real-kernel P2P / L2P: largest tail tile whose sources are split over lanes
 
   // ---- timing
@@ -504,6 +505,7 @@ void run_hadamards(xfmm_context* c, const M2LDev& m) {
     hb.nunion = c->m2l_blk_nu.ptr + b;
     hb.loc = c->m2l_blk_loc.ptr + size_t(b) * NM2L * 64;
     launch_hadamard(m, ops, c->f16_had ? c->m2l_hscale.ptr + size_t(L) * c->nfreq : nullptr, hb, c->tune_kch,
+                    c->tune_groups,
                     c->stream);
   }
 }
@@ -624,7 +626,7 @@ int xfmm_create(int kernel, int p, double wavek_re, double wavek_im, int precisi
     c->unit_surf.upload(u, c->stream);
     // the maps index the LDS cube of the transform kernels: must be a
     // one-to-one map into [0, p^3), onto the cube's surface points (the
-    // forward transform stores only those)
+    // forward transform stores only those, the inverse evaluates only those)
     for (int up = 0; up < 2; up++) {
       std::vector<int> m = local_map(p, up == 1);
       std::vector<char> seen(size_t(p) * p * p, 0);
@@ -632,7 +634,7 @@ int xfmm_create(int kernel, int p, double wavek_re, double wavek_im, int precisi
         if (v < 0 || v >= p * p * p || seen[v]) throw ArgError{"xfmm_create: surface-to-grid map is not one-to-one"};
         const int x = v % p, y = (v / p) % p, z = v / (p * p);
         const bool face = x == 0 || x == p - 1 || y == 0 || y == p - 1 || z == 0 || z == p - 1;
-        if (up && !face) throw ArgError{"xfmm_create: upward surface point inside the grid cube"};
+        if (!face) throw ArgError{"xfmm_create: surface point inside the grid cube"};
         seen[v] = 1;
       }
       (up ? c->map_up : c->map_dn).upload(m, c->stream);
@@ -1422,6 +1424,9 @@ int xfmm_set_tuning(xfmm_context* c, int param, int value) {
     } else if (param == XFMM_TUNE_P2P_SPLIT) {
       require(value >= 0 && value <= 32, "xfmm_set_tuning: P2P split threshold must be in [0, 32]");
       c->tune_split = value;
+    } else if (param == XFMM_TUNE_HAD_GROUPS) {
+      require(value == 1 || value == 2 || value == 4, "xfmm_set_tuning: Hadamard groups must be 1, 2 or 4");
+      c->tune_groups = value;
     } else {
       require(false, "xfmm_set_tuning: unknown parameter");
     }
@@ -1433,6 +1438,7 @@ int xfmm_get_tuning(const xfmm_context* c, int param, int* value) {
   if (!c || !value) return fail(nullptr, XFMM_ERR_ARG, "xfmm_get_tuning: null argument");
   if (param == XFMM_TUNE_HAD_KCH) *value = c->tune_kch;
   else if (param == XFMM_TUNE_P2P_SPLIT) *value = c->tune_split;
+  else if (param == XFMM_TUNE_HAD_GROUPS) *value = c->tune_groups;
   else return fail(const_cast<xfmm_context*>(c), XFMM_ERR_ARG, "xfmm_get_tuning: unknown parameter");
   return XFMM_OK;
 }
@@ -1505,17 +1511,23 @@ int xfmm_autotune(xfmm_context* c, int reps, double* best_ms) {
       if (m.amax) launch_zero(c->m2l_amax.ptr, 1, c->stream);
       launch_fft_up(m, t, c->m2l_src_node.ptr, c->n_m2l_src, c->stream);
       const int cand[] = {4, 8, 16, 32, 64};
-      int best = c->tune_kch;
+      // the groups matter only for the split-FP16 kernel
+      const int ngr = (c->f16_had && m.amax) ? 2 : 1;
+      int best = c->tune_kch, bestg = c->tune_groups;
       double bt = 1e30;
-      for (int v : cand) {
-        c->tune_kch = v;
-        const double ms = time_it([&] { run_hadamards(c, m); });
-        if (ms < bt) {
-          bt = ms;
-          best = v;
+      for (int gi = 0; gi < ngr; gi++)
+        for (int v : cand) {
+          c->tune_kch = v;
+          if (ngr > 1) c->tune_groups = gi == 0 ? 1 : 2;
+          const double ms = time_it([&] { run_hadamards(c, m); });
+          if (ms < bt) {
+            bt = ms;
+            best = v;
+            bestg = c->tune_groups;
+          }
         }
-      }
       c->tune_kch = best;
+      c->tune_groups = bestg;
       ms_kch = bt;
     }
     launch_zero(c->near.ptr, c->near.n, c->stream);

@@ -131,9 +131,11 @@ struct HadBlocks {
 // m2l_opsx: operators of the level in the expanded MFMA form of launch_m2l_expand
 // hscale != null (mixed, with m.amax set): split-FP16 operators of
 // launch_m2l_expand with their per-frequency scales, spectra scaled by *m.amax
-// kch: frequencies per workgroup (<= 0: HAD_KCH)
+// kch: frequencies per workgroup (<= 0: HAD_KCH); groups: 64-target blocks
+// per workgroup of the split-FP16 kernel (1, 2 or 4; they share the staged
+// operators of a frequency)
 void launch_hadamard(const M2LDev& m, const void* m2l_opsx, const float* hscale, const HadBlocks& hb, int kch,
-                     hipStream_t s);
+                     int groups, hipStream_t s);
 // Expand nlev operator levels from the reference layout [L][26][nfreq][64]
 // complex into the per-lane MFMA A-operand form [L][nfreq][26][64 lanes][4]
 // (float for the mixed path, else double), frequencies in the order of the

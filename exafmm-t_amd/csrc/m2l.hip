@@ -554,28 +554,34 @@ __global__ __launch_bounds__(256) void k_hadamard(M2LDev m, const R* __restrict_
 // matching 4 children (re, im) of its target's source, staged in LDS as
 // [term][h][union slot] 16-byte vectors.  Blocks, union staging, grid order
 // and result layout are those of k_hadamard.
-__global__ __launch_bounds__(256) void k_hadamard_h(M2LDev m, const h8* __restrict__ X,
-                                                    const float* __restrict__ hscale, HadBlocks hb, int nchunk,
-                                                    int kch) {
+template <int GPB>
+__global__ __launch_bounds__(256 * GPB) void k_hadamard_h(M2LDev m, const h8* __restrict__ X,
+                                                          const float* __restrict__ hscale, HadBlocks hb, int nchunk,
+                                                          int kch) {
   typedef float v4 __attribute__((ext_vector_type(4)));
   constexpr int US = HAD_UCAP + 1;
   constexpr int NP = NM2L / 2;
-  __shared__ h8 Us[4 * US];        // [term][h][union slot]; slot HAD_UCAP stays zero
-  __shared__ h8 Ms[NP * 2 * 64];   // [pair][term][lane]
+  constexpr int NT = 256 * GPB;     // threads: four waves per target block
+  __shared__ h8 Us[GPB][4 * US];    // per target block: [term][h][union slot]; slot HAD_UCAP stays zero
+  __shared__ h8 Ms[NP * 2 * 64];    // [pair][term][lane], shared by the GPB target blocks
+  const int nbg = (hb.nblk + GPB - 1) / GPB;
   const int w = blockIdx.x;
   const int x = w & 7, i = w >> 3;
-  const int chunk = (i / hb.nblk) * 8 + x;
-  const int b = i % hb.nblk;
+  const int chunk = (i / nbg) * 8 + x;
   if (chunk >= nchunk) return;
   const int kbeg = chunk * kch, kend = min(m.nfreq, kbeg + kch);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, sub = tid >> 8, st = tid & 255;
+  const int lane = tid & 63, wave = st >> 6;
   const int j = lane & 15, g = lane >> 4;
-  const int nu = hb.nunion[b];
-  if (tid < 4) Us[tid * US + HAD_UCAP] = h8{0, 0, 0, 0, 0, 0, 0, 0};
+  const int b = (i % nbg) * GPB + sub;
+  const bool live = b < hb.nblk;  // the last workgroup may hold fewer than GPB target blocks
+  const int nu = live ? hb.nunion[b] : 0;
+  h8* U = Us[sub];
+  if (st < 4) U[st * US + HAD_UCAP] = h8{0, 0, 0, 0, 0, 0, 0, 0};
   int uoff[NP];
 #pragma unroll
   for (int pm = 0; pm < NP; pm++) {
-    const int u = hb.loc[(size_t(b) * NM2L + 2 * pm + (g >> 1)) * 64 + wave * 16 + j];
+    const int u = live ? hb.loc[(size_t(b) * NM2L + 2 * pm + (g >> 1)) * 64 + wave * 16 + j] : -1;
     uoff[pm] = (g & 1) * US + (u >= 0 ? u : HAD_UCAP);
   }
   // spectrum scale: largest |value| of this evaluation's spectra -> [2^14, 2^15)
@@ -587,16 +593,18 @@ __global__ __launch_bounds__(256) void k_hadamard_h(M2LDev m, const h8* __restri
   typedef float f4 __attribute__((ext_vector_type(4)));
   const f4* fin = reinterpret_cast<const f4*>(m.fft_in);  // [slot][k][2 halves of 4 children]
   C2<float>* fout = reinterpret_cast<C2<float>*>(m.fft_out);
-  // staging: element e = tid + 256 q is union slot e / 2, children 4 (e % 2) .. + 3
+  // staging of a target block's union by its 256 threads: element
+  // e = st + 256 q is union slot e / 2, children 4 (e % 2) .. + 3
   constexpr int NU_Q = (HAD_UCAP * 2 + 255) / 256;
-  const int hh = tid & 1;
+  const int hh = st & 1;
   int usrc[NU_Q];
 #pragma unroll
   for (int q = 0; q < NU_Q; q++) {
-    const int e = tid + 256 * q;
+    const int e = st + 256 * q;
     usrc[q] = (e < nu * 2) ? hb.uni[size_t(b) * HAD_UCAP + (e >> 1)] : -1;
   }
-  constexpr int NM_Q = (NP * 2 * 64 + 255) / 256;
+  // the operators of a frequency, staged once for the GPB target blocks
+  constexpr int NM_Q = (NP * 2 * 64 + NT - 1) / NT;
   f4 ra[NU_Q], rb[NU_Q];
   h8 rm[NM_Q];
   auto fetch = [&](int k) {
@@ -610,9 +618,9 @@ __global__ __launch_bounds__(256) void k_hadamard_h(M2LDev m, const h8* __restri
     const h8* xk = X + size_t(k) * NP * 2 * 64;
 #pragma unroll
     for (int q = 0; q < NM_Q; q++)
-      if (tid + 256 * q < NP * 2 * 64) rm[q] = xk[tid + 256 * q];
+      if (tid + NT * q < NP * 2 * 64) rm[q] = xk[tid + NT * q];
   };
-  const int trg = hb.trg[size_t(b) * 64 + wave * 16 + j];
+  const int trg = live ? hb.trg[size_t(b) * 64 + wave * 16 + j] : -1;
   fetch(kbeg);
   for (int k = kbeg; k < kend; k++) {
     __syncthreads();
@@ -627,20 +635,20 @@ __global__ __launch_bounds__(256) void k_hadamard_h(M2LDev m, const h8* __restri
           hi[e] = h;
           lo[e] = _Float16(v - float(h));
         }
-        const int slot = (tid + 256 * q) >> 1;
-        Us[hh * US + slot] = hi;
-        Us[2 * US + hh * US + slot] = lo;
+        const int slot = (st + 256 * q) >> 1;
+        U[hh * US + slot] = hi;
+        U[2 * US + hh * US + slot] = lo;
       }
 #pragma unroll
     for (int q = 0; q < NM_Q; q++)
-      if (tid + 256 * q < NP * 2 * 64) Ms[tid + 256 * q] = rm[q];
+      if (tid + NT * q < NP * 2 * 64) Ms[tid + NT * q] = rm[q];
     __syncthreads();
     if (k + 1 < kend) fetch(k + 1);
     v4 ahh = {0, 0, 0, 0}, ahl = {0, 0, 0, 0}, alh = {0, 0, 0, 0};
 #pragma unroll
     for (int pm = 0; pm < NP; pm++) {
       const h8 ah = Ms[(2 * pm) * 64 + lane], al = Ms[(2 * pm + 1) * 64 + lane];
-      const h8 bh = Us[uoff[pm]], bl = Us[2 * US + uoff[pm]];
+      const h8 bh = U[uoff[pm]], bl = U[2 * US + uoff[pm]];
       ahh = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, ahh, 0, 0, 0);
       ahl = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, ahl, 0, 0, 0);
       alh = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, alh, 0, 0, 0);
@@ -705,6 +713,21 @@ __device__ __forceinline__ int surf_line_off(int z, int y) {
   if (z == P - 1) return P * P + (P - 2) * SL + y * P;
   const int b = P * P + (z - 1) * SL;
   return y == 0 ? b : (y == P - 1 ? b + P + 2 * (P - 2) : b + P + 2 * (y - 1));
+}
+// Face line f (0 <= f < 4p - 4) of the cube: lines (z, y) with z or y in
+// {0, p - 1}; interior lines are (1 + s / (p - 2), 1 + s % (p - 2)).
+template <int P>
+__device__ __forceinline__ void face_line(int f, int& z, int& y) {
+  if (f < P) {
+    z = 0;
+    y = f;
+  } else if (f < 2 * P) {
+    z = P - 1;
+    y = f - P;
+  } else {
+    z = 1 + ((f - 2 * P) >> 1);
+    y = ((f - 2 * P) & 1) ? P - 1 : 0;
+  }
 }
 template <int P>
 __device__ __forceinline__ int surf_compact(int v) {  // v = x + P (y + P z), a surface point
@@ -778,18 +801,9 @@ __global__ __launch_bounds__(256) void k_fft_up_mfma(M2LDev m, DevTree t, const 
       }
     };
     for (int e = tid; e < NCHILD * NFL; e += 256) {
-      const int c = e & 7, f = e >> 3;
+      const int c = e & 7;
       int z, y;
-      if (f < P) {
-        z = 0;
-        y = f;
-      } else if (f < 2 * P) {
-        z = P - 1;
-        y = f - P;
-      } else {
-        z = 1 + ((f - 2 * P) >> 1);
-        y = ((f - 2 * P) & 1) ? P - 1 : 0;
-      }
+      face_line<P>(e >> 3, z, y);
       const R* gl = gx + (c * NS + surf_line_off<P>(z, y)) * VW;
       C2<R> acc{0, 0};
 #pragma unroll
@@ -880,7 +894,9 @@ __global__ __launch_bounds__(256) void k_ifft_dn_mfma(M2LDev m, DevTree t, const
   constexpr int NCZ = 8 * n1, NTZ = NCZ / 16;        // z pass columns (ky, c)
   constexpr int NTZW = (NTZ + 3) / 4;                // z tiles per wave (at most)
   constexpr int NCY = 8 * P, NTY = (NCY + 15) / 16;  // y pass columns (z, c)
-  constexpr int NL = (NCHILD * P * P + 255) / 256;   // x lines per thread
+  constexpr int NFL = 4 * P - 4, NSL = (P - 2) * (P - 2);  // face lines, interior lines of the cube
+  constexpr int NLF = (NCHILD * NFL + 255) / 256;    // face lines per thread
+  constexpr int NLS = (NCHILD * NSL + 255) / 256;    // interior lines per thread
   extern __shared__ char smem_raw[];
   const int slot = blockIdx.x;
   if (slot >= nslots) return;
@@ -901,16 +917,24 @@ __global__ __launch_bounds__(256) void k_ifft_dn_mfma(M2LDev m, DevTree t, const
       const int f = mt * 8 + RowMap<R>::fa(j), v = (ks >> 1) * 4 + g;
       wf[mt][ks] = (f < P && v < n1) ? R(dft_entry(v * (P + f), n1, +1, RowMap<R>::ra(j), ks & 1)) : R(0);
     }
-  // x (c2r) results of this thread: the whole x line of y-pass outputs
-  // e = tid + 256 r = (z, y, c) (Ap index e), i.e. the grid points x in
-  // [p, 2p) of that line, accumulated over the kx slabs in registers
-  R gx[NL][P][VW];
+  // x results of this thread, accumulated over the kx slabs in registers:
+  // only the check surface is needed, i.e. the grid points x in [p, 2p) of
+  // the face lines (z or y in {p, 2p - 1}) and the two end points of the
+  // interior lines.  Face line items e = tid + 256 r (line e / 8, child e % 8)
+  // keep p values, interior line items two.
+  R gf[NLF][P][VW], gs[NLS][2][VW];
 #pragma unroll
-  for (int r = 0; r < NL; r++)
+  for (int r = 0; r < NLF; r++)
 #pragma unroll
     for (int x = 0; x < P; x++)
 #pragma unroll
-      for (int v = 0; v < VW; v++) gx[r][x][v] = R(0);
+      for (int v = 0; v < VW; v++) gf[r][x][v] = R(0);
+#pragma unroll
+  for (int r = 0; r < NLS; r++)
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+      for (int v = 0; v < VW; v++) gs[r][x][v] = R(0);
   const C2<RS>* in = reinterpret_cast<const C2<RS>*>(m.fft_out) + size_t(slot) * m.nfreq * NCHILD;
   // z-pass input fragments of this wave's tiles, loaded one slab ahead
   RS bz[NTZW][KS];
@@ -986,16 +1010,29 @@ __global__ __launch_bounds__(256) void k_ifft_dn_mfma(M2LDev m, DevTree t, const
       const C2<R> v = w[(kx * (P + x)) % n1];
       tw[x] = C2<R>{wt * v.x, wt * v.y};
     }
+    auto acc_x = [&](R* gv, const C2<R>& a, const C2<R>& tx) {
+      gv[0] += a.x * tx.x - a.y * tx.y;
+      if constexpr (CPLX) gv[VW - 1] += a.x * tx.y + a.y * tx.x;
+    };
 #pragma unroll
-    for (int r = 0; r < NL; r++) {
+    for (int r = 0; r < NLF; r++) {
       const int e = tid + 256 * r;
-      if (e < NCHILD * P * P) {
-        const C2<R> a = Ap[e];
+      if (e < NCHILD * NFL) {
+        int z, y;
+        face_line<P>(e >> 3, z, y);
+        const C2<R> a = Ap[(z * P + y) * NCHILD + (e & 7)];
 #pragma unroll
-        for (int x = 0; x < P; x++) {
-          gx[r][x][0] += a.x * tw[x].x - a.y * tw[x].y;
-          if constexpr (CPLX) gx[r][x][VW - 1] += a.x * tw[x].y + a.y * tw[x].x;
-        }
+        for (int x = 0; x < P; x++) acc_x(gf[r][x], a, tw[x]);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NLS; r++) {
+      const int e = tid + 256 * r;
+      if (e < NCHILD * NSL) {
+        const int sl = e >> 3, z = 1 + sl / (P - 2), y = 1 + sl % (P - 2);
+        const C2<R> a = Ap[(z * P + y) * NCHILD + (e & 7)];
+        acc_x(gs[r][0], a, tw[0]);
+        acc_x(gs[r][1], a, tw[P - 1]);
       }
     }
     // the next z pass writes Bp only (its readers are behind the barrier
@@ -1006,19 +1043,29 @@ __global__ __launch_bounds__(256) void k_ifft_dn_mfma(M2LDev m, DevTree t, const
   double* dn = t.dn + size_t(t.child0[node]) * ns * VW;
   // scatter the surface points of the lines (each surface point is one
   // grid point of one line)
+  auto put = [&](int c, int z, int y, int x, const R* gv) {
+    const int i = m.inv_dn[(z * P + y) * P + x];
+    if (i >= 0) {
 #pragma unroll
-  for (int r = 0; r < NL; r++) {
-    const int e = tid + 256 * r;
-    if (e >= NCHILD * P * P) continue;
-    const int c = e & 7, zy = e >> 3;
-#pragma unroll
-    for (int x = 0; x < P; x++) {
-      const int i = m.inv_dn[zy * P + x];
-      if (i >= 0) {
-#pragma unroll
-        for (int v = 0; v < VW; v++) dn[(size_t(c) * ns + i) * VW + v] += sc * double(gx[r][x][v]);
-      }
+      for (int v = 0; v < VW; v++) dn[(size_t(c) * ns + i) * VW + v] += sc * double(gv[v]);
     }
+  };
+#pragma unroll
+  for (int r = 0; r < NLF; r++) {
+    const int e = tid + 256 * r;
+    if (e >= NCHILD * NFL) continue;
+    int z, y;
+    face_line<P>(e >> 3, z, y);
+#pragma unroll
+    for (int x = 0; x < P; x++) put(e & 7, z, y, x, gf[r][x]);
+  }
+#pragma unroll
+  for (int r = 0; r < NLS; r++) {
+    const int e = tid + 256 * r;
+    if (e >= NCHILD * NSL) continue;
+    const int sl = e >> 3, z = 1 + sl / (P - 2), y = 1 + sl % (P - 2);
+    put(e & 7, z, y, 0, gs[r][0]);
+    put(e & 7, z, y, P - 1, gs[r][1]);
   }
 }
 
@@ -1128,15 +1175,22 @@ void launch_ifft_dn(const M2LDev& m, const DevTree& t, const int* trg_node, cons
 }
 
 void launch_hadamard(const M2LDev& m, const void* m2l_opsx, const float* hscale, const HadBlocks& hb, int kch,
-                     hipStream_t s) {
+                     int groups, hipStream_t s) {
   if (hb.nblk <= 0) return;
   if (kch < 1) kch = HAD_KCH;
   const int nchunk = (m.nfreq + kch - 1) / kch;
   const int nchunk8 = (nchunk + 7) / 8 * 8;
   dim3 grid(unsigned(nchunk8) * unsigned(hb.nblk));
-  if (m.mixed && hscale && m.amax)
-    hipLaunchKernelGGL(k_hadamard_h, grid, dim3(256), 0, s, m, static_cast<const h8*>(m2l_opsx), hscale, hb, nchunk,
-                       kch);
+  if (m.mixed && hscale && m.amax) {
+    auto go = [&](auto kern, int gp) {
+      dim3 gg(unsigned(nchunk8) * unsigned((hb.nblk + gp - 1) / gp));
+      hipLaunchKernelGGL(kern, gg, dim3(256 * gp), 0, s, m, static_cast<const h8*>(m2l_opsx), hscale, hb, nchunk, kch);
+    };
+    if (groups >= 4) go(k_hadamard_h<4>, 4);
+    else if (groups == 2) go(k_hadamard_h<2>, 2);
+    else go(k_hadamard_h<1>, 1);
+    return;
+  }
   else if (m.mixed)
     hipLaunchKernelGGL((k_hadamard<float>), grid, dim3(256), 0, s, m, static_cast<const float*>(m2l_opsx), hb, nchunk,
                        kch);

@@ -167,21 +167,24 @@ int xfmm_set_value_scale(xfmm_context* ctx, const double* scale);
 /* Launch-shape parameters of one context (no counterpart in the reference,
  * whose loops have fixed blocking; none of them changes which pairs are
  * summed):
- *   XFMM_TUNE_HAD_KCH   frequencies per Hadamard workgroup, 1..256 (default 16)
- *   XFMM_TUNE_P2P_SPLIT real kernels: P2P / L2P tail tiles of at most this many
- *                       targets (0..32) split each target's sources over
- *                       several lanes (default 32; the partial sums then add
- *                       in another order)
+ *   XFMM_TUNE_HAD_KCH    frequencies per Hadamard workgroup, 1..256 (default 16)
+ *   XFMM_TUNE_P2P_SPLIT  real kernels: P2P / L2P tail tiles of at most this many
+ *                        targets (0..32) split each target's sources over
+ *                        several lanes (default 32; the partial sums then add
+ *                        in another order)
+ *   XFMM_TUNE_HAD_GROUPS split-FP16 Hadamard (mixed / FLOAT): 64-target blocks
+ *                        per workgroup, 1, 2 or 4, which share the operators
+ *                        staged in LDS for a frequency (default 2)
  * Setting one drops a captured graph. */
-enum { XFMM_TUNE_HAD_KCH = 0, XFMM_TUNE_P2P_SPLIT = 1 };
+enum { XFMM_TUNE_HAD_KCH = 0, XFMM_TUNE_P2P_SPLIT = 1, XFMM_TUNE_HAD_GROUPS = 2 };
 int xfmm_set_tuning(xfmm_context* ctx, int param, int value);
 int xfmm_get_tuning(const xfmm_context* ctx, int param, int* value);
 
 /* Autotuner: on the uploaded tree and operators, times every candidate of
  * each tuning parameter (P2P split 0, 8, 16, 32 over the P2P + L2P kernels;
- * Hadamard chunk 4, 8, 16, 32, 64 over the Hadamard launches), fastest of
- * `reps` runs each (1..100), and keeps the fastest.  best_ms (may be NULL)
- * receives the two best times.  Leaves no evaluation result behind (call the
+ * Hadamard chunk 4, 8, 16, 32, 64 times groups 1, 2 over the Hadamard
+ * launches), fastest of `reps` runs each (1..100), and keeps the fastest.
+ * best_ms (may be NULL) receives the two best times.  Leaves no evaluation result behind (call the
  * passes again); drops a captured graph, so capture after tuning. */
 int xfmm_autotune(xfmm_context* ctx, int reps, double* best_ms);
 
