@@ -142,6 +142,7 @@ struct xfmm_context {
   // near-coincident P2P pairs ([4] ints each) corrected in FP64 after an
   // FP32 near field
   DevBuf<int> close_pairs, close_off;
+  DevBuf<double> close_coef;
   int nclose = 0;  // targets with near-coincident pairs
   DevBuf<double> src_rel, src_pack, surf_tab;
   DevBuf<float> surf_tab32;  // P2P sources relative to their leaf centre; packed (x, y, z, q)
@@ -198,6 +199,7 @@ struct xfmm_context {
     t.p2p_split = tune_split;
     t.close = close_pairs.ptr;
     t.close_off = close_off.ptr;
+    t.close_coef = close_coef.ptr;
     t.nclose = nclose;
     return t;
   }
@@ -418,10 +420,11 @@ void op_M2P(xfmm_context* c) {
 // once per tree: a first pass with room for ntrgs / 8 pairs, a second with
 // the exact count when that was too small; then sorted by (target, source)
 // and grouped per target on the host, so that the correction is applied in a
-// fixed order.
+// fixed order, and each pair's correction per unit charge computed once.
 void find_close_pairs(xfmm_context* c) {
   c->close_pairs.release();
   c->close_off.release();
+  c->close_coef.release();
   c->nclose = 0;
   if (!c->f32_p2p || c->n_p2p <= 0) return;
   DevBuf<int> count, found;
@@ -454,7 +457,9 @@ void find_close_pairs(xfmm_context* c) {
   off.push_back(n);
   c->close_pairs.upload(sorted, c->stream);
   c->close_off.upload(off, c->stream);
+  c->close_coef.alloc(size_t(n) * 4 * c->vw);
   c->nclose = int(off.size()) - 1;
+  launch_close_coef(c->kind, c->dev_tree(), c->wavek(), n, c->close_coef.ptr, c->stream);
   HIPCHK(hipStreamSynchronize(c->stream));
 }
 

@@ -39,6 +39,7 @@ struct DevTree {
   int p2p_split = 32;                 // real-kernel P2P / L2P: tail tiles of <= p2p_split targets (<= 32) split their sources over lanes
   const int* close = nullptr;         // near-coincident P2P pairs, [4] ints (target, source, target leaf, source leaf), sorted by target
   const int* close_off = nullptr;     // [nclose + 1]: pairs of the k-th target with any are close[close_off[k] .. close_off[k + 1])
+  const double* close_coef = nullptr; // [pairs][4 or 8]: FP64 minus FP32 value of each pair per unit charge
   int nclose = 0;                     // targets with near-coincident pairs
 };
 
@@ -50,8 +51,12 @@ enum TargetMode { MODE_P2P = 0, MODE_M2P = 1, MODE_L2P = 2 };
 // found (which may exceed cap: then call again with a larger buffer).
 // After an FP32 near field, launch_eval_targets replaces the FP32
 // contribution of each listed pair by its FP64 value (t.close, t.close_off,
-// t.nclose: the list sorted by target, grouped per target).
+// t.close_coef, t.nclose: the list sorted by target, grouped per target).
 void launch_close_find(const DevTree& t, const int* work, int nwork, int* out, int cap, int* count, hipStream_t s);
+// coef[k][4 or 8] of the sorted list t.close (npairs pairs): the pair's FP64
+// minus FP32 value per unit charge, FP32 operands formed as the near-field
+// kernel of `kind` forms them
+void launch_close_coef(int kind, const DevTree& t, Wavek k, int npairs, double* coef, hipStream_t s);
 enum SurfaceMode { MODE_P2M = 0, MODE_P2L = 1 };
 
 // Particle targets of the leaves in work[0..nwork): P2P (sources: leaf

@@ -650,23 +650,30 @@ void launch_close_find(const DevTree& t, const int* work, int nwork, int* out, i
   hipLaunchKernelGGL(k_close_find, dim3(nwork), dim3(64), 0, s, t, work, out, cap, count);
 }
 
-// One thread per target with listed pairs, its pairs in list order (so the
-// result does not depend on scheduling): trg_value += sum of (FP64 - FP32
-// contribution) / 4 pi.  The FP32 operands are formed as in k_p2p_smem
-// (PACKED, real kernels: packed source relative to its leaf centre, target
-// moved into the source leaf's frame) or k_eval_targets (both relative to
-// the target leaf centre).
+// The correction of each listed pair per unit charge, once per tree: one
+// thread per pair, coef[k][a] = FP64 minus FP32 value of the pair with q = 1
+// (real kernels: a = potential, gradient; Helmholtz: their (re, im) parts).
+// The FP32 operands are formed as the near-field kernel forms them: PACKED
+// (real kernels, k_p2p_smem) the source relative to its leaf centre as
+// k_pack_sources rounds it and the target moved into the source leaf's frame,
+// else (k_eval_targets) both relative to the target leaf centre.  A pair is
+// linear in its charge, so per evaluation its correction is q coef (complex
+// product for Helmholtz); the FP32 value of a charged pair differs from q
+// times that of a unit charge by the rounding of the products, ~1e-7 of the
+// pair's value.
 template <int KIND, bool PACKED>
-__device__ __forceinline__ void close_pair_fix(const DevTree& t, Wavek wk, const int* cp, double* fix) {
+__global__ __launch_bounds__(256) void k_close_coef(DevTree t, Wavek wk, int npairs, double* __restrict__ coef) {
   constexpr bool C = kcplx<KIND>();
   constexpr int NA = C ? 8 : 4;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= npairs) return;
+  const int* cp = t.close + 4 * size_t(k);
   const int it = cp[0], is = cp[1], tl = cp[2], sl = cp[3];
   const double cx = t.center[3 * tl], cy = t.center[3 * tl + 1], cz = t.center[3 * tl + 2];
   const double* x = t.trg + 3 * size_t(it);
   const double* y = t.src + 3 * size_t(is);
   const float tx = float(x[0] - cx), ty = float(x[1] - cy), tz = float(x[2] - cz);
   const float kr = float(wk.re), ki = float(wk.im);
-  const double qr = t.q[size_t(is) * t.vw], qi = C ? t.q[size_t(is) * t.vw + 1] : 0.0;
   float a32[NA];
   double a64[NA];
 #pragma unroll
@@ -674,38 +681,62 @@ __device__ __forceinline__ void close_pair_fix(const DevTree& t, Wavek wk, const
     a32[a] = 0.f;
     a64[a] = 0.0;
   }
-  float dx, dy, dz, q32;
+  float dx, dy, dz;
   if constexpr (PACKED) {
-    const float4 v = static_cast<const float4*>(t.src_pack)[is];
-    dx = v.x - (tx - float(t.center[3 * sl] - cx));
-    dy = v.y - (ty - float(t.center[3 * sl + 1] - cy));
-    dz = v.z - (tz - float(t.center[3 * sl + 2] - cz));
-    q32 = v.w;
+    const double* rel = t.src_rel + 3 * size_t(is);
+    dx = float(rel[0]) - (tx - float(t.center[3 * sl] - cx));
+    dy = float(rel[1]) - (ty - float(t.center[3 * sl + 1] - cy));
+    dz = float(rel[2]) - (tz - float(t.center[3 * sl + 2] - cz));
   } else {
     dx = float(y[0] - cx) - tx;
     dy = float(y[1] - cy) - ty;
     dz = float(y[2] - cz) - tz;
-    q32 = float(qr);
   }
   // a leaf with itself: the near-field kernel left out the pairs with FP32
   // r^2 <= rmin2 (their FP32 value is 0 here too)
-  if (tl == sl) pair_grad<KIND, float, RSQ_MASK>(dx, dy, dz, q32, float(qi), kr, ki, a32, close_rmin2<float>(t, tl));
-  else pair_grad<KIND, float, RSQ_SEP>(dx, dy, dz, q32, float(qi), kr, ki, a32);
-  pair_grad<KIND, double, RSQ_MASK>(y[0] - x[0], y[1] - x[1], y[2] - x[2], qr, qi, wk.re, wk.im, a64);
+  if (tl == sl) pair_grad<KIND, float, RSQ_MASK>(dx, dy, dz, 1.f, 0.f, kr, ki, a32, close_rmin2<float>(t, tl));
+  else pair_grad<KIND, float, RSQ_SEP>(dx, dy, dz, 1.f, 0.f, kr, ki, a32);
+  pair_grad<KIND, double, RSQ_MASK>(y[0] - x[0], y[1] - x[1], y[2] - x[2], 1.0, 0.0, wk.re, wk.im, a64);
 #pragma unroll
-  for (int a = 0; a < NA; a++) fix[a] += a64[a] - double(a32[a]);
+  for (int a = 0; a < NA; a++) coef[size_t(k) * NA + a] = a64[a] - double(a32[a]);
 }
 
-template <int KIND, bool PACKED>
-__global__ __launch_bounds__(256) void k_close_fix(DevTree t, Wavek wk) {
-  constexpr int NA = kcplx<KIND>() ? 8 : 4;
+void launch_close_coef(int kind, const DevTree& t, Wavek k, int npairs, double* coef, hipStream_t s) {
+  if (npairs <= 0) return;
+  const dim3 g((npairs + 255) / 256), b(256);
+  if (kind == LAPLACE) hipLaunchKernelGGL((k_close_coef<LAPLACE, true>), g, b, 0, s, t, k, npairs, coef);
+  else if (kind == YUKAWA) hipLaunchKernelGGL((k_close_coef<YUKAWA, true>), g, b, 0, s, t, k, npairs, coef);
+  else hipLaunchKernelGGL((k_close_coef<HELMHOLTZ, false>), g, b, 0, s, t, k, npairs, coef);
+}
+
+// One thread per target with listed pairs, its pairs in list order (so the
+// result does not depend on scheduling): trg_value += sum of q_s coef / 4 pi.
+template <int KIND>
+__global__ __launch_bounds__(256) void k_close_fix(DevTree t) {
+  constexpr bool C = kcplx<KIND>();
+  constexpr int NA = C ? 8 : 4;
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
   if (u >= t.nclose) return;
   double fix[NA];
 #pragma unroll
   for (int a = 0; a < NA; a++) fix[a] = 0.0;
   const int k0 = t.close_off[u], k1 = t.close_off[u + 1];
-  for (int k = k0; k < k1; k++) close_pair_fix<KIND, PACKED>(t, wk, t.close + 4 * size_t(k), fix);
+  for (int k = k0; k < k1; k++) {
+    const int is = t.close[4 * size_t(k) + 1];
+    const double* cf = t.close_coef + size_t(k) * NA;
+    const double qr = t.q[size_t(is) * t.vw];
+    if constexpr (C) {
+      const double qi = t.q[size_t(is) * t.vw + 1];
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+        fix[2 * c] += qr * cf[2 * c] - qi * cf[2 * c + 1];
+        fix[2 * c + 1] += qr * cf[2 * c + 1] + qi * cf[2 * c];
+      }
+    } else {
+#pragma unroll
+      for (int a = 0; a < NA; a++) fix[a] += qr * cf[a];
+    }
+  }
   double* o = t.trg_value + size_t(t.close[4 * size_t(k0)]) * 4 * t.vw;
 #pragma unroll
   for (int a = 0; a < NA; a++) o[a] += fix[a] * FOUR_PI_INV;
@@ -729,11 +760,8 @@ static void eval_targets_t(int mode, const DevTree& t, Wavek k, const int* work,
   else hipLaunchKernelGGL((k_eval_targets<KIND, R, MODE_L2P>), dim3(nwork), dim3(64), 0, s, t, k, work);
   // FP32 near field: the near-coincident pairs in FP64 (same stream, after it)
   if constexpr (sizeof(R) == 4) {
-    if (mode == MODE_P2P && t.nclose > 0 && t.close && t.close_off) {
-      const dim3 g((t.nclose + 255) / 256);
-      if (real && t.src_pack && t.src_rel) hipLaunchKernelGGL((k_close_fix<KIND, true>), g, dim3(256), 0, s, t, k);
-      else hipLaunchKernelGGL((k_close_fix<KIND, false>), g, dim3(256), 0, s, t, k);
-    }
+    if (mode == MODE_P2P && t.nclose > 0 && t.close && t.close_off && t.close_coef)
+      hipLaunchKernelGGL(k_close_fix<KIND>, dim3((t.nclose + 255) / 256), dim3(256), 0, s, t);
   }
 }
 
