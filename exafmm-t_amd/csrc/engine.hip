@@ -75,6 +75,14 @@ struct xfmm_context {
   // FP32 MFMA GEMMs: the pseudo-inverse solves of P2M / L2P (FLOAT mode only)
   // and the M2M / L2L translations (both need the float-truncated operators)
   bool f32_pinv = false, f32_trans = false;
+  // mixed path: the current charges' net sum exceeds a tenth of their
+  // absolute sum (one-signed charges).  Their potentials are then large
+  // against their gradients, and the FP32 rounding of M2M / L2L and of the
+  // transforms' arithmetic, relative to the potential, is not small against
+  // the gradient: those stages run in FP64 while this holds.
+  bool net_heavy = false;
+  bool trans32() const { return f32_trans && !net_heavy; }
+  bool fft32() const { return f32_m2l_fft && !net_heavy; }
   // dn and trg_value are zero (cleared by the last upward pass or clear_values
   // and not written since), so the downward pass need not clear them itself
   bool down_clean = false;
@@ -96,6 +104,7 @@ struct xfmm_context {
   bool capturing = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
+  bool graph_stale = false;  // captured with the kernels of the other charge class (net_heavy)
   std::string err;
 
   // ---- operators
@@ -143,6 +152,7 @@ struct xfmm_context {
   // FP32 near field
   DevBuf<int> close_pairs, close_off;
   DevBuf<double> close_coef;
+  DevBuf<double> charge_sums;  // partial sums of the charge balance rule (device charges)
   int nclose = 0;  // targets with near-coincident pairs
   DevBuf<double> src_rel, src_pack, surf_tab;
   DevBuf<float> surf_tab32;  // P2P sources relative to their leaf centre; packed (x, y, z, q)
@@ -211,7 +221,7 @@ struct xfmm_context {
     m.nsurf = nsurf;
     m.cplx = cplx;
     m.mixed = f32_m2l;
-    m.mixed_fft = f32_m2l_fft;
+    m.mixed_fft = fft32();
     m.map_up = map_up.ptr;
     m.map_dn = map_dn.ptr;
     m.inv_dn = inv_dn.ptr;
@@ -301,6 +311,7 @@ void drop_graph(xfmm_context* c) {
   if (c->graph) HIPCHK(hipGraphDestroy(c->graph));
   c->graph_exec = nullptr;
   c->graph = nullptr;
+  c->graph_stale = false;
 }
 
 // ---------------------------------------------------------------- operators
@@ -366,8 +377,8 @@ void op_M2M(xfmm_context* c) {
     if (e <= b) continue;
     const int L = c->op_level(l);
     // parent row = [child_0 .. child_7] (contiguous, K = 8 nsurf) x M2Mcat
-    launch_gemm_rows(c->f32_trans, e - b, c->nsurf * vw, 8 * c->nsurf * vw, c->up.ptr, c->nl_child_rows.ptr + b,
-                     gemm_op(c->f32_trans, c, c->m2mcat, c->m2mcat32, L * blk), c->up.ptr, c->nl_rows.ptr + b, 1.0, 1,
+    launch_gemm_rows(c->trans32(), e - b, c->nsurf * vw, 8 * c->nsurf * vw, c->up.ptr, c->nl_child_rows.ptr + b,
+                     gemm_op(c->trans32(), c, c->m2mcat, c->m2mcat32, L * blk), c->up.ptr, c->nl_rows.ptr + b, 1.0, 1,
                      c->stream, c->gemm_scratch.ptr, c->gemm_scratch.n);
   }
 }
@@ -379,8 +390,8 @@ void op_L2L(xfmm_context* c) {
     const int b = c->nl_level_off[l], e = c->nl_level_off[l + 1];
     if (e <= b) continue;
     const int L = c->op_level(l);
-    launch_gemm_rows(c->f32_trans, e - b, 8 * c->nsurf * vw, c->nsurf * vw, c->dn.ptr, c->nl_rows.ptr + b,
-                     gemm_op(c->f32_trans, c, c->l2lcat, c->l2lcat32, L * blk), c->dn.ptr, c->nl_child_rows.ptr + b,
+    launch_gemm_rows(c->trans32(), e - b, 8 * c->nsurf * vw, c->nsurf * vw, c->dn.ptr, c->nl_rows.ptr + b,
+                     gemm_op(c->trans32(), c, c->l2lcat, c->l2lcat32, L * blk), c->dn.ptr, c->nl_child_rows.ptr + b,
                      1.0, 1, c->stream, c->gemm_scratch.ptr, c->gemm_scratch.n);
   }
 }
@@ -1180,7 +1191,19 @@ int xfmm_set_tree(xfmm_context* c, const xfmm_tree* t) {
     HIPCHK(hipStreamSynchronize(st));
     c->have_tree = true;
     c->down_clean = false;
+    c->net_heavy = false;  // zero charges
   });
+}
+
+// Mixed path: classify the charges by |sum q| / sum |q| (the net_heavy rule
+// above; sums = {Re sum, Im sum, sum |q|}).  A captured graph holds the
+// kernels of the class it was captured with: a change of class marks it for
+// recapture at its next launch.
+static void classify_charges(xfmm_context* c, const double* sums) {
+  if (c->prec != XFMM_MIXED) return;
+  const bool heavy = std::hypot(sums[0], sums[1]) > 0.1 * sums[2];
+  if (heavy != c->net_heavy && c->graph_exec) c->graph_stale = !c->graph_stale;
+  c->net_heavy = heavy;
 }
 
 int xfmm_set_charges_host(xfmm_context* c, const double* q) {
@@ -1191,7 +1214,15 @@ int xfmm_set_charges_host(xfmm_context* c, const double* q) {
     if (c->near_launched) HIPCHK(hipStreamWaitEvent(c->stream, c->ev[20], 0));
     c->near_pending = false;
     if (c->nsrcs) HIPCHK(hipMemcpyAsync(c->q.ptr, q, size_t(c->nsrcs) * c->vw * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    double sums[3] = {0, 0, 0};
+    for (size_t i = 0; i < size_t(c->nsrcs); i++) {
+      const double re = q[i * c->vw], im = c->vw == 2 ? q[2 * i + 1] : 0.0;
+      sums[0] += re;
+      sums[1] += im;
+      sums[2] += std::hypot(re, im);
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
+    classify_charges(c, sums);
   });
 }
 
@@ -1203,6 +1234,17 @@ int xfmm_set_charges_device(xfmm_context* c, const void* q) {
     if (c->near_launched) HIPCHK(hipStreamWaitEvent(c->stream, c->ev[20], 0));
     c->near_pending = false;
     if (c->nsrcs) HIPCHK(hipMemcpyAsync(c->q.ptr, q, size_t(c->nsrcs) * c->vw * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    if (c->prec == XFMM_MIXED) {
+      double sums[3] = {0, 0, 0};
+      if (c->nsrcs) {
+        c->charge_sums.alloc(3 * (CHARGE_SUM_BLOCKS + 1));
+        launch_charge_sums(c->q.ptr, c->nsrcs, c->vw == 2, c->charge_sums.ptr, c->stream);
+        HIPCHK(hipMemcpyAsync(sums, c->charge_sums.ptr + 3 * CHARGE_SUM_BLOCKS, sizeof(sums), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+      }
+      classify_charges(c, sums);
+    }
   });
 }
 
@@ -1308,6 +1350,7 @@ int xfmm_graph_capture(xfmm_context* c) {
 int xfmm_graph_launch(xfmm_context* c) {
   return guarded(c, [&] {
     require(c->graph_exec != nullptr, "xfmm_graph_launch: no graph (call xfmm_graph_capture)");
+    if (c->graph_stale && xfmm_graph_capture(c) != XFMM_OK) throw ArgError{c->err};
     // the graph zeroes and recomputes everything it produces; eager state
     // afterwards is that of a completed evaluation
     HIPCHK(hipGraphLaunch(c->graph_exec, c->stream));

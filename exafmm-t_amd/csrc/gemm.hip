@@ -303,6 +303,40 @@ void launch_axpy(double* y, const double* x, size_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_axpy, dim3(unsigned(blocks)), dim3(256), 0, s, y, x, n);
 }
 
+__global__ __launch_bounds__(256) void k_charge_sums(const double* __restrict__ q, int n, bool cplx,
+                                                     double* __restrict__ out) {
+  __shared__ double red[3][256];
+  double a = 0, b = 0, c = 0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += CHARGE_SUM_BLOCKS * 256) {
+    const double re = cplx ? q[2 * size_t(i)] : q[i], im = cplx ? q[2 * size_t(i) + 1] : 0.0;
+    a += re;
+    b += im;
+    c += sqrt(re * re + im * im);
+  }
+  red[0][threadIdx.x] = a;
+  red[1][threadIdx.x] = b;
+  red[2][threadIdx.x] = c;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int k = 0; k < 3; k++) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) out[3 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_charge_sums_total(double* __restrict__ out) {
+  if (threadIdx.x >= 3) return;
+  double t = 0;
+  for (int b = 0; b < CHARGE_SUM_BLOCKS; b++) t += out[3 * b + threadIdx.x];
+  out[3 * CHARGE_SUM_BLOCKS + threadIdx.x] = t;
+}
+
+void launch_charge_sums(const double* q, int n, bool cplx, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_charge_sums, dim3(CHARGE_SUM_BLOCKS), dim3(256), 0, s, q, n, cplx, out);
+  hipLaunchKernelGGL(k_charge_sums_total, dim3(1), dim3(64), 0, s, out);
+}
+
 __global__ void k_scale_values(double* __restrict__ val, const double* __restrict__ scale, int n, int vw) {
   const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   const size_t w = size_t(4) * vw;

@@ -158,6 +158,11 @@ size_t m2l_lds_bytes(const M2LDev& m);
 
 void launch_zero(double* p, size_t n, hipStream_t s);
 void launch_axpy(double* y, const double* x, size_t n, hipStream_t s);  // y += x
+// {Re sum q, Im sum q, sum |q|} of n charges (complex: interleaved pairs),
+// in a fixed order: per-block partial sums in out[0 .. 3 CHARGE_SUM_BLOCKS),
+// their total in out[3 CHARGE_SUM_BLOCKS .. + 3)
+constexpr int CHARGE_SUM_BLOCKS = 256;
+void launch_charge_sums(const double* q, int n, bool cplx, double* out, hipStream_t s);
 // values T[n][4]: value 0 (potential) times scale[i][0], values 1..3
 // (gradient) times scale[i][1]; vw doubles per value
 void launch_scale_values(double* val, const double* scale, int n, int vw, hipStream_t s);

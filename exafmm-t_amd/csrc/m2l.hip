@@ -221,9 +221,12 @@ static size_t fft_node_lds(const M2LDev& m, size_t rb) {
   return fwd > inv ? fwd : inv;
 }
 
-// per-node transforms: p in [3, 12], LDS within the CU's 160 KiB
+// per-node transforms: p in [3, 12], LDS within the CU's 160 KiB with FP64
+// arithmetic (so that the choice, and with it the frequency order of the
+// spectra and expanded operators, does not depend on the arithmetic type,
+// which the mixed path switches with the charges)
 static bool fft_node_ok(const M2LDev& m) {
-  return m.p >= 3 && m.p <= 12 && fft_node_lds(m, (m.mixed && m.mixed_fft) ? 4 : 8) <= 160 * 1024;
+  return m.p >= 3 && m.p <= 12 && fft_node_lds(m, 8) <= 160 * 1024;
 }
 
 template <typename F>

@@ -94,7 +94,11 @@ typedef struct {
  * for this mode: those directions only amplify FP32 rounding.  The environment
  * variable XFMM_MIXED_STAGES (comma list of p2p, m2l, had16, fft, surf, far,
  * trans, pinv) overrides the FP32 stage set of XFMM_MIXED when a context is
- * created.  XFMM_FLOAT is the counterpart of the reference's -DFLOAT build
+ * created.  Charges whose net sum exceeds a tenth of their absolute sum
+ * (|sum q| > 0.1 sum |q|, e.g. charges of one sign) make the potential large
+ * against its gradient; while such charges are set, XFMM_MIXED runs M2M / L2L
+ * and the transforms' arithmetic in FP64 (their FP32 rounding, relative to
+ * the potential, would otherwise reach ~1e-5 of the gradient).  XFMM_FLOAT is the counterpart of the reference's -DFLOAT build
  * (include/exafmm_t.h:21-37): every evaluation stage computes in FP32,
  * including the pseudo-inverse solves (FP32 MFMA); values stay FP64 in memory.
  * It expects the same truncated operators: with untruncated pseudo-inverses
@@ -124,7 +128,10 @@ int xfmm_set_operators(xfmm_context* ctx, int nlevels, const double* uc2e_u, con
 int xfmm_set_tree(xfmm_context* ctx, const xfmm_tree* tree);
 
 /* Sorted charges, T[nsrcs].  Host pointer (..._host) or device pointer
- * already resident in HBM (..._device, same layout). */
+ * already resident in HBM (..._device, same layout; XFMM_MIXED sums them on
+ * the device and waits for the two sums).  XFMM_MIXED classifies the charges
+ * by |sum q| / sum |q| (see xfmm_create); after a change of class the next
+ * xfmm_graph_launch captures the graph again. */
 int xfmm_set_charges_host(xfmm_context* ctx, const double* q);
 int xfmm_set_charges_device(xfmm_context* ctx, const void* q_device);
 
@@ -149,8 +156,10 @@ int xfmm_downward_pass(xfmm_context* ctx);
  * runs and leaves the same values as an eager evaluation.  For callers that
  * evaluate many times, or many small problems (one graph per context, all
  * replayed back to back); no counterpart in the reference, whose evaluation
- * is host-driven.  The graph is recaptured by calling capture again, and
- * dropped by xfmm_set_tree / xfmm_set_operators.  Per-operator timings are
+ * is host-driven.  The graph is recaptured by calling capture again (and by
+ * the next launch itself after charges of the other balance class were set,
+ * XFMM_MIXED, see xfmm_create), and dropped by xfmm_set_tree /
+ * xfmm_set_operators.  Per-operator timings are
  * not recorded by replays. */
 int xfmm_graph_capture(xfmm_context* ctx);
 int xfmm_graph_launch(xfmm_context* ctx);

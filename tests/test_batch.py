@@ -164,4 +164,5 @@ def test_batch_mixed_vs_oracle(require_gpu):
         e = (rel_err(vals[i][:, 0], ref[:, 0]), rel_err(vals[i][:, 1:], ref[:, 1:]))
         print("mixed problem %d (N=%d) vs oracle (truncated operators) phi %.2e grad %.2e" % (i, sizes[i], *e),
               flush=True)
-        assert e[0] < 1e-5 and e[1] < 1e-4
+        # charges 1/N (one sign): M2M / L2L and the transforms run in FP64 on the mixed path
+        assert e[0] < 1e-5 and e[1] < 1e-5

@@ -82,6 +82,11 @@ def test_laplace_headline_parameters(require_gpu):
 
 
 def test_laplace_p8_full_size(require_gpu):
+    """Config 1 at its full size, charges U[0,1): FP64 against the oracle, and the mixed path within 1e-5 of
+    the FP64 direct sum for potential and gradient.  The charges are of one sign, so the mixed path runs
+    M2M / L2L and the transforms' arithmetic in FP64 (with them in FP32 the gradient error was 1.09e-5
+    at the same parameters: their rounding is relative to the potential, which one-signed charges make
+    large against the gradient)."""
     pr = make_problem("laplace", 100000, 8, ncrit=64, dist="cube", seed=41, charge="u01")
     print("laplace p=8 ncrit=64 uniform N=100000", lists(pr), flush=True)
     orc = O.Oracle(pr["kid"], 8, pr["flat"], pr["ops"], pr["wavek"])
@@ -98,6 +103,14 @@ def test_laplace_p8_full_size(require_gpu):
     print("fp64 vs oracle phi %.2e grad %.2e; vs direct phi %.2e grad %.2e" % (e + ed), flush=True)
     assert e[0] < FP64_TOL and e[1] < FP64_TOL
     assert ed[0] < 1e-6 and ed[1] < 1e-5
+    eng = _engine(pr, abi.MIXED, truncated_ops(pr))
+    eng.upward_pass()
+    eng.downward_pass()
+    vm = eng.values()
+    eng.close()
+    em = errs(vm[idx], d)
+    print("mixed vs direct phi %.2e grad %.2e" % em, flush=True)
+    assert em[0] < MIXED_TOL and em[1] < MIXED_TOL
 
 
 def test_yukawa_sphere_adaptive(require_gpu):

@@ -342,12 +342,15 @@ def test_linearity_and_repeatability(require_gpu):
                                               ("helmholtz", abi.MIXED), ("laplace", abi.FLOAT)])
 def test_graph_replay_matches_eager(require_gpu, kernel, precision):
     """A captured evaluation replays to the eager values, replay after replay, with eager evaluations and new
-    charges in between (the graph holds kernels only: its clears are fill kernels)."""
+    charges in between (the graph holds kernels only: its clears are fill kernels).  The last charge set is
+    of one sign: the mixed path then runs M2M / L2L and the transforms in FP64, and a replay after the
+    change of class captures the graph again by itself."""
     pr = make_problem(kernel, 6000, 6, ncrit=32, dist="plummer", seed=21, precision=precision)
     print("graph case", kernel, precision, flush=True)
     eng = _engine(pr, precision)
     rng = np.random.default_rng(8)
     qs = [pr["qs"]] + [pr["qs"] * (rng.random(pr["qs"].size) + 0.5) for _ in range(3)]
+    qs.append((np.abs(pr["qs"]) + 0.1).astype(pr["qs"].dtype))
     eager = []
     for q in qs:
         eng.set_charges(q)
