@@ -130,7 +130,11 @@ def p2p_pairs(flat):
     return int(np.sum(per_target.astype(np.int64) * flat["trg_count"]))
 
 
-def run_path(prob, p, precision, steps, warmup):
+def run_path(prob, p, precision, steps, warmup, graph=True):
+    """Times `steps` evaluations (upward + downward pass) after `warmup` eager ones.  With graph=True a
+    step replays the evaluation captured as one HIP graph (the same kernels, without the per-operator
+    timing events and launch gaps); the per-operator times come from one extra eager, event-timed
+    evaluation before the timed region."""
     from exafmm import abi
     eng = abi.Engine(abi.LAPLACE, p, 0.0, precision, 0)
     eng.set_operators(operators_for(prob, precision))
@@ -141,19 +145,33 @@ def run_path(prob, p, precision, steps, warmup):
         eng.upward_pass()
         eng.downward_pass()
     eng.synchronize()
+    eng.upward_pass()
+    eng.downward_pass()
+    eng.synchronize()
+    kt = eng.timings()
+    if graph:
+        eng.graph_capture()
+        eng.graph_launch()
+        eng.synchronize()
+
+    def step():
+        if graph:
+            eng.graph_launch()
+        else:
+            eng.upward_pass()
+            eng.downward_pass()
+
     per_step = []
     t0 = time.perf_counter()
     for _ in range(steps):
         ts = time.perf_counter()
-        eng.upward_pass()
-        eng.downward_pass()
+        step()
         eng.synchronize()
         per_step.append(time.perf_counter() - ts)
     total = time.perf_counter() - t0
-    kt = eng.timings()
     vals = eng.values()
     eng.close()
-    return dict(total=total, per_step=per_step, kernels=kt, values=vals)
+    return dict(total=total, per_step=per_step, kernels=kt, values=vals, graph=graph)
 
 
 def sample_error(prob, values, nsample, seed):
@@ -515,6 +533,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--mixed-only", action="store_true", help="time the headline mixed-precision path only")
+    ap.add_argument("--eager", action="store_true",
+                    help="time eager launches (with per-operator events) instead of replays of the captured graph")
     ap.add_argument("--workload", default="laplace_uniform",
                     choices=["laplace_uniform", "laplace_batched"] + sorted(WORKLOADS))
     a = ap.parse_args()
@@ -550,7 +570,7 @@ def main():
         paths = [PATHS[0]]
     for name, prec in paths:
         log("timing %s path: warmup %d steps %d" % (name, a.warmup, a.steps))
-        r = run_path(prob, a.p, prec, a.steps, a.warmup)
+        r = run_path(prob, a.p, prec, a.steps, a.warmup, graph=not a.eager)
         r["err"] = sample_error(prob, r["values"], a.nsample, seed=123)
         log("%s: %.3f ms/step err phi %.2e grad %.2e kernels %s" % (
             name, 1e3 * r["total"] / a.steps, r["err"][0], r["err"][1],
@@ -595,6 +615,8 @@ def main():
                    "N": a.n, "p": a.p, "ncrit": a.ncrit, "depth": prob["depth"], "nodes": int(flat["nnodes"]),
                    "p2p_pairs": npairs, "m2l_pairs": m2l_pairs,
                    "precision": PRECISION_LABEL[hname]},
+        "timing": ("eager launches" if a.eager else "replays of the evaluation captured as one HIP graph") +
+                  "; kernels_ms from one event-timed eager evaluation",
         "err_phi": head["err"][0],
         "err_grad": head["err"][1],
         "kernels_ms": kt,

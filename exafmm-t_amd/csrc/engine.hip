@@ -163,6 +163,7 @@ struct xfmm_context {
   // tuning (xfmm_set_tuning / xfmm_autotune)
   int tune_kch = HAD_KCH;  // Hadamard frequencies per workgroup
   int tune_groups = 2;     // split-FP16 Hadamard: 64-target blocks per workgroup
+  int tune_tiles = 1;      // split-FP16 Hadamard: 16-target tiles per wave
   int tune_split = 32;     // real-kernel P2P / L2P: largest tail tile whose sources are split over lanes
 
   // ---- timing
@@ -322,7 +323,7 @@ void op_P2L(xfmm_context* c);
 void op_M2P(xfmm_context* c);
 void op_P2P(xfmm_context* c);
 void op_L2L(xfmm_context* c);
-void op_L2P(xfmm_context* c);
+void op_L2P(xfmm_context* c, const double* near_add = nullptr, bool wait_near = false);
 
 void check_ready(xfmm_context* c) {
   if (!c->have_tree) throw ArgError{"xfmm: tree not set (call xfmm_set_tree)"};
@@ -396,7 +397,10 @@ void op_L2L(xfmm_context* c) {
   }
 }
 
-void op_L2P(xfmm_context* c) {
+// near_add (may be null): per-target values added to the L2P result (the
+// near field of a full evaluation); wait_near: they come from the second
+// stream, joined before the L2P kernel.
+void op_L2P(xfmm_context* c, const double* near_add, bool wait_near) {
   const size_t ns2 = gemm_op_size(c);
   const int vw = c->vw;
   for (int l = 0; l <= c->depth; l++) {
@@ -412,6 +416,8 @@ void op_L2P(xfmm_context* c) {
                      c->leaf_scale(l), 0, c->stream);
   }
   DevTree t = c->dev_tree();
+  if (wait_near) HIPCHK(hipStreamWaitEvent(c->stream, c->ev[20], 0));
+  t.near_add = near_add;
   launch_eval_targets(c->kind, c->f32_far, MODE_L2P, t, c->wavek(), c->w_l2p_eval.ptr, c->n_l2p_eval, c->t_l2p.ptr,
                       c->nt_l2p, c->stream);
 }
@@ -521,8 +527,7 @@ void run_hadamards(xfmm_context* c, const M2LDev& m) {
     hb.nunion = c->m2l_blk_nu.ptr + b;
     hb.loc = c->m2l_blk_loc.ptr + size_t(b) * NM2L * 64;
     launch_hadamard(m, ops, c->f16_had ? c->m2l_hscale.ptr + size_t(L) * c->nfreq : nullptr, hb, c->tune_kch,
-                    c->tune_groups,
-                    c->stream);
+                    c->tune_groups, c->tune_tiles, c->stream);
   }
 }
 
@@ -1310,10 +1315,10 @@ int xfmm_downward_pass(xfmm_context* c) {
     record(c, 7);
     op_L2L(c);
     record(c, 8);
-    op_L2P(c);
+    // every leaf with targets has an L2P work item, so the L2P kernel adds
+    // the near field to each target's result exactly once
+    op_L2P(c, c->near.ptr, pend);
     record(c, 9);
-    if (pend) HIPCHK(hipStreamWaitEvent(c->stream, c->ev[20], 0));
-    launch_axpy(c->trg_value.ptr, c->near.ptr, c->trg_value.n, c->stream);
     if (c->vscale.ptr) launch_scale_values(c->trg_value.ptr, c->vscale.ptr, c->ntrgs, c->vw, c->stream);
     c->near_pending = false;
     c->near_overlapped = pend;
@@ -1475,6 +1480,9 @@ int xfmm_set_tuning(xfmm_context* c, int param, int value) {
     } else if (param == XFMM_TUNE_HAD_GROUPS) {
       require(value == 1 || value == 2 || value == 4, "xfmm_set_tuning: Hadamard groups must be 1, 2 or 4");
       c->tune_groups = value;
+    } else if (param == XFMM_TUNE_HAD_TILES) {
+      require(value == 1 || value == 2, "xfmm_set_tuning: Hadamard tiles per wave must be 1 or 2");
+      c->tune_tiles = value;
     } else {
       require(false, "xfmm_set_tuning: unknown parameter");
     }
@@ -1487,6 +1495,7 @@ int xfmm_get_tuning(const xfmm_context* c, int param, int* value) {
   if (param == XFMM_TUNE_HAD_KCH) *value = c->tune_kch;
   else if (param == XFMM_TUNE_P2P_SPLIT) *value = c->tune_split;
   else if (param == XFMM_TUNE_HAD_GROUPS) *value = c->tune_groups;
+  else if (param == XFMM_TUNE_HAD_TILES) *value = c->tune_tiles;
   else return fail(const_cast<xfmm_context*>(c), XFMM_ERR_ARG, "xfmm_get_tuning: unknown parameter");
   return XFMM_OK;
 }
@@ -1559,23 +1568,28 @@ int xfmm_autotune(xfmm_context* c, int reps, double* best_ms) {
       if (m.amax) launch_zero(c->m2l_amax.ptr, 1, c->stream);
       launch_fft_up(m, t, c->m2l_src_node.ptr, c->n_m2l_src, c->stream);
       const int cand[] = {4, 8, 16, 32, 64};
-      // the groups matter only for the split-FP16 kernel
-      const int ngr = (c->f16_had && m.amax) ? 2 : 1;
-      int best = c->tune_kch, bestg = c->tune_groups;
+      // groups and tiles per wave matter only for the split-FP16 kernel
+      const int ngr = (c->f16_had && m.amax) ? 4 : 1;
+      int best = c->tune_kch, bestg = c->tune_groups, bestt = c->tune_tiles;
       double bt = 1e30;
       for (int gi = 0; gi < ngr; gi++)
         for (int v : cand) {
           c->tune_kch = v;
-          if (ngr > 1) c->tune_groups = gi == 0 ? 1 : 2;
+          if (ngr > 1) {
+            c->tune_groups = (gi & 1) ? 2 : 1;
+            c->tune_tiles = (gi & 2) ? 2 : 1;
+          }
           const double ms = time_it([&] { run_hadamards(c, m); });
           if (ms < bt) {
             bt = ms;
             best = v;
             bestg = c->tune_groups;
+            bestt = c->tune_tiles;
           }
         }
       c->tune_kch = best;
       c->tune_groups = bestg;
+      c->tune_tiles = bestt;
       ms_kch = bt;
     }
     launch_zero(c->near.ptr, c->near.n, c->stream);

@@ -36,6 +36,8 @@ struct DevTree {
   double* up = nullptr;               // T[nnodes][nsurf]
   double* dn = nullptr;               // T[nnodes][nsurf]
   double* trg_value = nullptr;        // T[ntrgs][4]
+  const double* near_add = nullptr;   // L2P only (may be null): [ntrgs][4] values that the L2P kernels add to
+                                      // each target's result after their own sum (the evaluation's near field)
   int p2p_split = 32;                 // real-kernel P2P / L2P: tail tiles of <= p2p_split targets (<= 32) split their sources over lanes
   const int* close = nullptr;         // near-coincident P2P pairs, [4] ints (target, source, target leaf, source leaf), sorted by target
   const int* close_off = nullptr;     // [nclose + 1]: pairs of the k-th target with any are close[close_off[k] .. close_off[k + 1])
@@ -138,9 +140,10 @@ struct HadBlocks {
 // launch_m2l_expand with their per-frequency scales, spectra scaled by *m.amax
 // kch: frequencies per workgroup (<= 0: HAD_KCH); groups: 64-target blocks
 // per workgroup of the split-FP16 kernel (1, 2 or 4; they share the staged
-// operators of a frequency)
+// operators of a frequency); tiles: 16-target tiles per wave of that kernel
+// (1 or 2; a wave's tiles share the operator operands it reads)
 void launch_hadamard(const M2LDev& m, const void* m2l_opsx, const float* hscale, const HadBlocks& hb, int kch,
-                     int groups, hipStream_t s);
+                     int groups, int tiles, hipStream_t s);
 // Expand nlev operator levels from the reference layout [L][26][nfreq][64]
 // complex into the per-lane MFMA A-operand form [L][nfreq][26][64 lanes][4]
 // (float for the mixed path, else double), frequencies in the order of the

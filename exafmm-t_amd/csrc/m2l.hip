@@ -557,14 +557,16 @@ __global__ __launch_bounds__(256) void k_hadamard(M2LDev m, const R* __restrict_
 // matching 4 children (re, im) of its target's source, staged in LDS as
 // [term][h][union slot] 16-byte vectors.  Blocks, union staging, grid order
 // and result layout are those of k_hadamard.
-template <int GPB>
-__global__ __launch_bounds__(256 * GPB) void k_hadamard_h(M2LDev m, const h8* __restrict__ X,
-                                                          const float* __restrict__ hscale, HadBlocks hb, int nchunk,
-                                                          int kch) {
+template <int GPB, int TPW>
+__global__ __launch_bounds__(256 * GPB / TPW) void k_hadamard_h(M2LDev m, const h8* __restrict__ X,
+                                                                const float* __restrict__ hscale, HadBlocks hb,
+                                                                int nchunk, int kch) {
   typedef float v4 __attribute__((ext_vector_type(4)));
   constexpr int US = HAD_UCAP + 1;
   constexpr int NP = NM2L / 2;
-  constexpr int NT = 256 * GPB;     // threads: four waves per target block
+  static_assert(TPW == 1 || TPW == 2, "tiles per wave");
+  constexpr int TPB = 256 / TPW;    // threads per target block: 4 / TPW waves of TPW 16-target tiles
+  constexpr int NT = TPB * GPB;
   __shared__ h8 Us[GPB][4 * US];    // per target block: [term][h][union slot]; slot HAD_UCAP stays zero
   __shared__ h8 Ms[NP * 2 * 64];    // [pair][term][lane], shared by the GPB target blocks
   const int nbg = (hb.nblk + GPB - 1) / GPB;
@@ -573,7 +575,7 @@ __global__ __launch_bounds__(256 * GPB) void k_hadamard_h(M2LDev m, const h8* __
   const int chunk = (i / nbg) * 8 + x;
   if (chunk >= nchunk) return;
   const int kbeg = chunk * kch, kend = min(m.nfreq, kbeg + kch);
-  const int tid = threadIdx.x, sub = tid >> 8, st = tid & 255;
+  const int tid = threadIdx.x, sub = tid / TPB, st = tid % TPB;
   const int lane = tid & 63, wave = st >> 6;
   const int j = lane & 15, g = lane >> 4;
   const int b = (i % nbg) * GPB + sub;
@@ -581,11 +583,18 @@ __global__ __launch_bounds__(256 * GPB) void k_hadamard_h(M2LDev m, const h8* __
   const int nu = live ? hb.nunion[b] : 0;
   h8* U = Us[sub];
   if (st < 4) U[st * US + HAD_UCAP] = h8{0, 0, 0, 0, 0, 0, 0, 0};
-  int uoff[NP];
+  // tile t of this wave: targets (TPW wave + t) 16 + j of the block
+  int uoff[TPW][NP];
+  int trg[TPW];
 #pragma unroll
-  for (int pm = 0; pm < NP; pm++) {
-    const int u = live ? hb.loc[(size_t(b) * NM2L + 2 * pm + (g >> 1)) * 64 + wave * 16 + j] : -1;
-    uoff[pm] = (g & 1) * US + (u >= 0 ? u : HAD_UCAP);
+  for (int t = 0; t < TPW; t++) {
+    const int tt = (TPW * wave + t) * 16 + j;
+#pragma unroll
+    for (int pm = 0; pm < NP; pm++) {
+      const int u = live ? hb.loc[(size_t(b) * NM2L + 2 * pm + (g >> 1)) * 64 + tt] : -1;
+      uoff[t][pm] = (g & 1) * US + (u >= 0 ? u : HAD_UCAP);
+    }
+    trg[t] = live ? hb.trg[size_t(b) * 64 + tt] : -1;
   }
   // spectrum scale: largest |value| of this evaluation's spectra -> [2^14, 2^15)
   const float amax = __uint_as_float(*m.amax);
@@ -596,14 +605,14 @@ __global__ __launch_bounds__(256 * GPB) void k_hadamard_h(M2LDev m, const h8* __
   typedef float f4 __attribute__((ext_vector_type(4)));
   const f4* fin = reinterpret_cast<const f4*>(m.fft_in);  // [slot][k][2 halves of 4 children]
   C2<float>* fout = reinterpret_cast<C2<float>*>(m.fft_out);
-  // staging of a target block's union by its 256 threads: element
-  // e = st + 256 q is union slot e / 2, children 4 (e % 2) .. + 3
-  constexpr int NU_Q = (HAD_UCAP * 2 + 255) / 256;
+  // staging of a target block's union by its TPB threads: element
+  // e = st + TPB q is union slot e / 2, children 4 (e % 2) .. + 3
+  constexpr int NU_Q = (HAD_UCAP * 2 + TPB - 1) / TPB;
   const int hh = st & 1;
   int usrc[NU_Q];
 #pragma unroll
   for (int q = 0; q < NU_Q; q++) {
-    const int e = st + 256 * q;
+    const int e = st + TPB * q;
     usrc[q] = (e < nu * 2) ? hb.uni[size_t(b) * HAD_UCAP + (e >> 1)] : -1;
   }
   // the operators of a frequency, staged once for the GPB target blocks
@@ -623,7 +632,6 @@ __global__ __launch_bounds__(256 * GPB) void k_hadamard_h(M2LDev m, const h8* __
     for (int q = 0; q < NM_Q; q++)
       if (tid + NT * q < NP * 2 * 64) rm[q] = xk[tid + NT * q];
   };
-  const int trg = live ? hb.trg[size_t(b) * 64 + wave * 16 + j] : -1;
   fetch(kbeg);
   for (int k = kbeg; k < kend; k++) {
     __syncthreads();
@@ -638,7 +646,7 @@ __global__ __launch_bounds__(256 * GPB) void k_hadamard_h(M2LDev m, const h8* __
           hi[e] = h;
           lo[e] = _Float16(v - float(h));
         }
-        const int slot = (st + 256 * q) >> 1;
+        const int slot = (st + TPB * q) >> 1;
         U[hh * US + slot] = hi;
         U[2 * US + hh * US + slot] = lo;
       }
@@ -647,20 +655,30 @@ __global__ __launch_bounds__(256 * GPB) void k_hadamard_h(M2LDev m, const h8* __
       if (tid + NT * q < NP * 2 * 64) Ms[tid + NT * q] = rm[q];
     __syncthreads();
     if (k + 1 < kend) fetch(k + 1);
-    v4 ahh = {0, 0, 0, 0}, ahl = {0, 0, 0, 0}, alh = {0, 0, 0, 0};
+    v4 ahh[TPW], ahl[TPW], alh[TPW];
+#pragma unroll
+    for (int t = 0; t < TPW; t++) ahh[t] = ahl[t] = alh[t] = v4{0, 0, 0, 0};
 #pragma unroll
     for (int pm = 0; pm < NP; pm++) {
+      // the operator pair is read once and applied to the wave's TPW tiles
       const h8 ah = Ms[(2 * pm) * 64 + lane], al = Ms[(2 * pm + 1) * 64 + lane];
-      const h8 bh = U[uoff[pm]], bl = U[2 * US + uoff[pm]];
-      ahh = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, ahh, 0, 0, 0);
-      ahl = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, ahl, 0, 0, 0);
-      alh = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, alh, 0, 0, 0);
+#pragma unroll
+      for (int t = 0; t < TPW; t++) {
+        const h8 bh = U[uoff[t][pm]], bl = U[2 * US + uoff[t][pm]];
+        ahh[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, ahh[t], 0, 0, 0);
+        ahl[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, ahl[t], 0, 0, 0);
+        alh[t] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, alh[t], 0, 0, 0);
+      }
     }
     const float sc = hscale[k] * isb;
-    const v4 acc = (ahh + (ahl + alh)) * sc;
     // lane (g, j) register q holds row 4 g + q of target j: (Re, Im) of
     // children 2g and 2g + 1, one 16-byte store
-    if (trg >= 0) *reinterpret_cast<v4*>(fout + (size_t(trg) * m.nfreq + k) * NCHILD + 2 * g) = acc;
+#pragma unroll
+    for (int t = 0; t < TPW; t++) {
+      const v4 acc = (ahh[t] + (ahl[t] + alh[t])) * sc;
+      if (trg[t] >= 0)
+        *reinterpret_cast<v4*>(fout + (size_t(trg[t]) * m.nfreq + k) * NCHILD + 2 * g) = acc;
+    }
   }
 }
 
@@ -1178,20 +1196,27 @@ void launch_ifft_dn(const M2LDev& m, const DevTree& t, const int* trg_node, cons
 }
 
 void launch_hadamard(const M2LDev& m, const void* m2l_opsx, const float* hscale, const HadBlocks& hb, int kch,
-                     int groups, hipStream_t s) {
+                     int groups, int tiles, hipStream_t s) {
   if (hb.nblk <= 0) return;
   if (kch < 1) kch = HAD_KCH;
   const int nchunk = (m.nfreq + kch - 1) / kch;
   const int nchunk8 = (nchunk + 7) / 8 * 8;
   dim3 grid(unsigned(nchunk8) * unsigned(hb.nblk));
   if (m.mixed && hscale && m.amax) {
-    auto go = [&](auto kern, int gp) {
+    auto go = [&](auto kern, int gp, int tpw) {
       dim3 gg(unsigned(nchunk8) * unsigned((hb.nblk + gp - 1) / gp));
-      hipLaunchKernelGGL(kern, gg, dim3(256 * gp), 0, s, m, static_cast<const h8*>(m2l_opsx), hscale, hb, nchunk, kch);
+      hipLaunchKernelGGL(kern, gg, dim3(256 * gp / tpw), 0, s, m, static_cast<const h8*>(m2l_opsx), hscale, hb,
+                         nchunk, kch);
     };
-    if (groups >= 4) go(k_hadamard_h<4>, 4);
-    else if (groups == 2) go(k_hadamard_h<2>, 2);
-    else go(k_hadamard_h<1>, 1);
+    if (tiles >= 2) {
+      if (groups >= 4) go(k_hadamard_h<4, 2>, 4, 2);
+      else if (groups == 2) go(k_hadamard_h<2, 2>, 2, 2);
+      else go(k_hadamard_h<1, 2>, 1, 2);
+    } else {
+      if (groups >= 4) go(k_hadamard_h<4, 1>, 4, 1);
+      else if (groups == 2) go(k_hadamard_h<2, 1>, 2, 1);
+      else go(k_hadamard_h<1, 1>, 1, 1);
+    }
     return;
   }
   else if (m.mixed)

@@ -20,8 +20,14 @@
 
 namespace xfmm {
 
+// Helmholtz with a real wave number (Im k = 0): the evaluation kernels are
+// instantiated without the damping factor e^{-Im(k) r}, which is then exactly
+// 1 (the values are bit-identical to the general instantiation).  Internal to
+// the launchers of this file.
+constexpr int HELMHOLTZ_REAL = 3;
+
 template <int KIND>
-constexpr bool kcplx() { return KIND == HELMHOLTZ; }
+constexpr bool kcplx() { return KIND == HELMHOLTZ || KIND == HELMHOLTZ_REAL; }
 
 // 1/r of a pair.  RSQ_LIB: library rsqrt, r = 0 gives 0 (the direct sum used as
 // accuracy check).  RSQ_MASK: hardware estimate refined to full precision
@@ -60,7 +66,7 @@ __device__ __forceinline__ void pair_grad(R dx, R dy, R dz, R qr, R qi, R kr, R 
     a[3] += s * dz;
   } else {
     const R r = r2 * invr;
-    const R e = dexp(-ki * r) * invr;
+    const R e = KIND == HELMHOLTZ_REAL ? invr : dexp(-ki * r) * invr;
     R sn, cs;
     dsincos(kr * r, &sn, &cs);
     const R gr = e * cs, gi = e * sn;
@@ -68,7 +74,7 @@ __device__ __forceinline__ void pair_grad(R dx, R dy, R dz, R qr, R qi, R kr, R 
     const R pim = qr * gi + qi * gr;
     a[0] += pr;
     a[1] += pim;
-    const R cgr = (R(1) + ki * r) * invr * invr;
+    const R cgr = (KIND == HELMHOLTZ_REAL ? invr : (R(1) + ki * r) * invr) * invr;
     const R cgi = -kr * invr;
     const R fr = cgr * pr - cgi * pim;
     const R fi = cgr * pim + cgi * pr;
@@ -92,7 +98,7 @@ __device__ __forceinline__ void pair_pot(R dx, R dy, R dz, R qr, R qi, R kr, R k
     a[0] += qr * dexp(-kr * (r2 * invr)) * invr;
   } else {
     const R r = r2 * invr;
-    const R e = dexp(-ki * r) * invr;
+    const R e = KIND == HELMHOLTZ_REAL ? invr : dexp(-ki * r) * invr;
     R sn, cs;
     dsincos(kr * r, &sn, &cs);
     const R gr = e * cs, gi = e * sn;
@@ -191,8 +197,11 @@ __global__ void k_pack_sources(int n, const double* __restrict__ rel, const doub
 // lane (target lane % nt, phase lane / nt) takes every k-th source from its
 // phase on (vector loads); the k partial sums of a target are added in FP64
 // through LDS by split_reduce.
+// add (may be null): NA values per target added after the sum (the
+// near field that an L2P adds to its result, see DevTree::near_add).
 template <typename R, int NA>
-__device__ __forceinline__ void split_reduce(const R* acc, const R* comp, int nt, int k, double* out) {
+__device__ __forceinline__ void split_reduce(const R* acc, const R* comp, int nt, int k, double* out,
+                                             const double* add = nullptr) {
   __shared__ double red[TILE][NA];
   const int lane = threadIdx.x;
 #pragma unroll
@@ -203,7 +212,8 @@ __device__ __forceinline__ void split_reduce(const R* acc, const R* comp, int nt
     for (int a = 0; a < NA; a++) {
       double v = 0;
       for (int ph = 0; ph < k; ph++) v += red[lane + ph * nt][a];
-      out[a] += v * FOUR_PI_INV;
+      if (add) out[a] = (out[a] + v * FOUR_PI_INV) + add[a];
+      else out[a] += v * FOUR_PI_INV;
     }
   }
 }
@@ -362,7 +372,8 @@ __global__ __launch_bounds__(64) void k_l2p_smem(DevTree t, Wavek wk, const int*
         }
       fold_tile<R, NA>(acc, comp, part);
     }
-    split_reduce<R, NA>(acc, comp, nt, k, o + 4 * size_t(lane < nt ? lane : 0));
+    const size_t ol = 4 * size_t(lane < nt ? lane : 0);
+    split_reduce<R, NA>(acc, comp, nt, k, o + ol, t.near_add ? t.near_add + size_t(tb + t0) * 4 + ol : nullptr);
     return;
   }
   for (int s0 = 0; s0 < ns; s0 += TILE) {
@@ -383,8 +394,13 @@ __global__ __launch_bounds__(64) void k_l2p_smem(DevTree t, Wavek wk, const int*
     fold_tile<R, NA>(acc, comp, part);
   }
   if (act) {
+    const double* add = t.near_add ? t.near_add + size_t(tb + t0) * 4 + 4 * lane : nullptr;
 #pragma unroll
-    for (int a = 0; a < NA; a++) o[4 * lane + a] += (double(acc[a]) - double(comp[a])) * FOUR_PI_INV;
+    for (int a = 0; a < NA; a++) {
+      const double v = (double(acc[a]) - double(comp[a])) * FOUR_PI_INV;
+      if (add) o[4 * lane + a] = (o[4 * lane + a] + v) + add[a];
+      else o[4 * lane + a] += v;
+    }
   }
 }
 
@@ -469,8 +485,13 @@ __global__ __launch_bounds__(64) void k_eval_targets(DevTree t, Wavek wk, const 
     }
     if (act) {
       double* o = t.trg_value + size_t(tb + ti) * 4 * vw;
+      const double* add = (MODE == MODE_L2P && t.near_add) ? t.near_add + size_t(tb + ti) * 4 * vw : nullptr;
 #pragma unroll
-      for (int a = 0; a < NA; a++) o[a] += (double(acc[a]) - double(comp[a])) * FOUR_PI_INV;
+      for (int a = 0; a < NA; a++) {
+        const double v = (double(acc[a]) - double(comp[a])) * FOUR_PI_INV;
+        if (add) o[a] = (o[a] + v) + add[a];
+        else o[a] += v;
+      }
     }
   }
 }
@@ -748,7 +769,7 @@ template <int KIND, typename R>
 static void eval_targets_t(int mode, const DevTree& t, Wavek k, const int* work, int nwork, const int* tiles,
                            int ntiles, hipStream_t s) {
   if (nwork <= 0) return;
-  const bool real = KIND != HELMHOLTZ && tiles && ntiles > 0;
+  const bool real = !kcplx<KIND>() && tiles && ntiles > 0;
   if (mode == MODE_P2P && real && t.src_pack && t.src_rel) {
     if (t.nsrcs > 0)
       hipLaunchKernelGGL(k_pack_sources<R>, dim3((t.nsrcs + 255) / 256), dim3(256), 0, s, t.nsrcs, t.src_rel, t.q,
@@ -761,7 +782,8 @@ static void eval_targets_t(int mode, const DevTree& t, Wavek k, const int* work,
   // FP32 near field: the near-coincident pairs in FP64 (same stream, after it)
   if constexpr (sizeof(R) == 4) {
     if (mode == MODE_P2P && t.nclose > 0 && t.close && t.close_off && t.close_coef)
-      hipLaunchKernelGGL(k_close_fix<KIND>, dim3((t.nclose + 255) / 256), dim3(256), 0, s, t);
+      hipLaunchKernelGGL(k_close_fix<KIND == HELMHOLTZ_REAL ? int(HELMHOLTZ) : KIND>, dim3((t.nclose + 255) / 256),
+                         dim3(256), 0, s, t);
   }
 }
 
@@ -772,8 +794,12 @@ void launch_eval_targets(int kind, bool f32, int mode, const DevTree& t, Wavek k
                       : eval_targets_t<LAPLACE, double>(mode, t, k, work, nwork, tiles, ntiles, s); break;
     case YUKAWA: f32 ? eval_targets_t<YUKAWA, float>(mode, t, k, work, nwork, tiles, ntiles, s)
                      : eval_targets_t<YUKAWA, double>(mode, t, k, work, nwork, tiles, ntiles, s); break;
-    default: f32 ? eval_targets_t<HELMHOLTZ, float>(mode, t, k, work, nwork, tiles, ntiles, s)
-                 : eval_targets_t<HELMHOLTZ, double>(mode, t, k, work, nwork, tiles, ntiles, s); break;
+    default:
+      if (k.im == 0.0) f32 ? eval_targets_t<HELMHOLTZ_REAL, float>(mode, t, k, work, nwork, tiles, ntiles, s)
+                           : eval_targets_t<HELMHOLTZ_REAL, double>(mode, t, k, work, nwork, tiles, ntiles, s);
+      else f32 ? eval_targets_t<HELMHOLTZ, float>(mode, t, k, work, nwork, tiles, ntiles, s)
+               : eval_targets_t<HELMHOLTZ, double>(mode, t, k, work, nwork, tiles, ntiles, s);
+      break;
   }
 }
 
@@ -792,8 +818,12 @@ void launch_eval_surface(int kind, bool f32, int mode, const DevTree& t, Wavek k
                       : eval_surface_t<LAPLACE, double>(mode, t, k, work, nwork, check, s); break;
     case YUKAWA: f32 ? eval_surface_t<YUKAWA, float>(mode, t, k, work, nwork, check, s)
                      : eval_surface_t<YUKAWA, double>(mode, t, k, work, nwork, check, s); break;
-    default: f32 ? eval_surface_t<HELMHOLTZ, float>(mode, t, k, work, nwork, check, s)
-                 : eval_surface_t<HELMHOLTZ, double>(mode, t, k, work, nwork, check, s); break;
+    default:
+      if (k.im == 0.0) f32 ? eval_surface_t<HELMHOLTZ_REAL, float>(mode, t, k, work, nwork, check, s)
+                           : eval_surface_t<HELMHOLTZ_REAL, double>(mode, t, k, work, nwork, check, s);
+      else f32 ? eval_surface_t<HELMHOLTZ, float>(mode, t, k, work, nwork, check, s)
+               : eval_surface_t<HELMHOLTZ, double>(mode, t, k, work, nwork, check, s);
+      break;
   }
 }
 

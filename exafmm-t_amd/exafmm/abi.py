@@ -12,7 +12,7 @@ import numpy as np
 
 LAPLACE, HELMHOLTZ, MODIFIED_HELMHOLTZ = 0, 1, 2
 FP64, MIXED, FLOAT = 0, 1, 2
-TUNE_HAD_KCH, TUNE_P2P_SPLIT, TUNE_HAD_GROUPS = 0, 1, 2
+TUNE_HAD_KCH, TUNE_P2P_SPLIT, TUNE_HAD_GROUPS, TUNE_HAD_TILES = 0, 1, 2, 3
 LIB_PATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lib",
                         "libexafmm_t_amd.so")
 TIMING_NAMES = ["P2M", "M2M", "P2L", "M2P", "P2P", "M2L", "L2L", "L2P", "FFT", "HAD", "IFFT"]
@@ -274,7 +274,7 @@ class Engine:
         best = np.zeros(2)
         self._check(self.lib.xfmm_autotune(self.h, int(reps), _ptr(best)), "xfmm_autotune")
         return ({"had_kch": self.get_tuning(TUNE_HAD_KCH), "p2p_split": self.get_tuning(TUNE_P2P_SPLIT),
-                 "had_groups": self.get_tuning(TUNE_HAD_GROUPS)},
+                 "had_groups": self.get_tuning(TUNE_HAD_GROUPS), "had_tiles": self.get_tuning(TUNE_HAD_TILES)},
                 {"p2p_l2p_ms": float(best[0]), "hadamard_ms": float(best[1])})
 
     def timings(self):

@@ -184,15 +184,18 @@ int xfmm_set_value_scale(xfmm_context* ctx, const double* scale);
  *   XFMM_TUNE_HAD_GROUPS split-FP16 Hadamard (mixed / FLOAT): 64-target blocks
  *                        per workgroup, 1, 2 or 4, which share the operators
  *                        staged in LDS for a frequency (default 2)
+ *   XFMM_TUNE_HAD_TILES  split-FP16 Hadamard: 16-target tiles per wave, 1 or 2;
+ *                        a wave applies each operator operand it reads from
+ *                        LDS to all of its tiles (default 1)
  * Setting one drops a captured graph. */
-enum { XFMM_TUNE_HAD_KCH = 0, XFMM_TUNE_P2P_SPLIT = 1, XFMM_TUNE_HAD_GROUPS = 2 };
+enum { XFMM_TUNE_HAD_KCH = 0, XFMM_TUNE_P2P_SPLIT = 1, XFMM_TUNE_HAD_GROUPS = 2, XFMM_TUNE_HAD_TILES = 3 };
 int xfmm_set_tuning(xfmm_context* ctx, int param, int value);
 int xfmm_get_tuning(const xfmm_context* ctx, int param, int* value);
 
 /* Autotuner: on the uploaded tree and operators, times every candidate of
  * each tuning parameter (P2P split 0, 8, 16, 32 over the P2P + L2P kernels;
- * Hadamard chunk 4, 8, 16, 32, 64 times groups 1, 2 over the Hadamard
- * launches), fastest of `reps` runs each (1..100), and keeps the fastest.
+ * Hadamard chunk 4, 8, 16, 32, 64 times groups 1, 2 times tiles per wave 1, 2
+ * over the Hadamard launches), fastest of `reps` runs each (1..100), and keeps the fastest.
  * best_ms (may be NULL) receives the two best times.  Leaves no evaluation result behind (call the
  * passes again); drops a captured graph, so capture after tuning. */
 int xfmm_autotune(xfmm_context* ctx, int reps, double* best_ms);

@@ -262,14 +262,17 @@ def test_mixed_hadamard_fp16_vs_fp32(require_gpu, monkeypatch, kernel, p, dist):
 @pytest.mark.parametrize("kernel,p", [("laplace", 8), ("helmholtz", 5)])
 def test_hadamard_tuning_same_values(require_gpu, kernel, p):
     """The launch-shape parameters change no value: the split-FP16 Hadamard with 1, 2 or 4 target blocks
-    per workgroup and 4 or 32 frequencies per workgroup gives bit-identical results (each target's sum is
-    computed by the same lanes in the same order), and the autotuner picks a valid setting."""
+    per workgroup, 1 or 2 target tiles per wave and 4 or 32 frequencies per workgroup gives bit-identical
+    results (each target's sum is computed by the same lanes in the same order), and the autotuner picks a
+    valid setting."""
     pr = make_problem(kernel, 20000, p, ncrit=64, dist="cube", seed=12, precision=abi.MIXED)
     eng = _engine(pr, abi.MIXED)
     out = []
-    for groups, kch in ((1, 16), (2, 16), (4, 16), (2, 4), (2, 32)):
+    for groups, kch, tiles in ((1, 16, 1), (2, 16, 1), (4, 16, 1), (2, 4, 1), (2, 32, 1), (1, 16, 2), (2, 16, 2),
+                               (4, 8, 2)):
         eng.set_tuning(abi.TUNE_HAD_GROUPS, groups)
         eng.set_tuning(abi.TUNE_HAD_KCH, kch)
+        eng.set_tuning(abi.TUNE_HAD_TILES, tiles)
         eng.upward_pass()
         eng.downward_pass()
         out.append(eng.values())
@@ -278,12 +281,15 @@ def test_hadamard_tuning_same_values(require_gpu, kernel, p):
     tuning, ms = eng.autotune(2)
     print("autotune", tuning, ms, flush=True)
     assert tuning["had_groups"] in (1, 2) and tuning["had_kch"] in (4, 8, 16, 32, 64)
+    assert tuning["had_tiles"] in (1, 2)
     eng.set_tuning(abi.TUNE_P2P_SPLIT, 32)  # the near field's split changes its summation order
     eng.upward_pass()
     eng.downward_pass()
     assert np.array_equal(eng.values(), out[0])
     with pytest.raises(abi.XfmmError):
         eng.set_tuning(abi.TUNE_HAD_GROUPS, 3)
+    with pytest.raises(abi.XfmmError):
+        eng.set_tuning(abi.TUNE_HAD_TILES, 4)
     eng.close()
 
 
