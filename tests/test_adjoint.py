@@ -251,3 +251,92 @@ def test_torch_autograd(require_gpu, kernel):
     a.close()
     assert np.array_equal(qt.grad.numpy(), gq)
     assert np.array_equal(yt.grad.numpy(), gy)
+
+
+_DEVICE_SCRIPT = r"""
+import sys
+import torch  # first: the engine then binds to the HIP runtime that torch loaded
+sys.path.insert(0, sys.argv[1])
+import numpy as np
+from exafmm.autograd import fmm_potential, hip_runtimes
+
+kernel, wavek = sys.argv[2], complex(sys.argv[3])
+rng = np.random.default_rng(8)
+n = 3000
+y = rng.normal(size=(n, 3))
+y /= np.linalg.norm(y, axis=1)[:, None]
+x = rng.random((n, 3)) * 0.5
+if kernel == "helmholtz":
+    q = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+    w = rng.normal(size=n) + 1j * rng.normal(size=n)
+else:
+    wavek = wavek.real
+    q = rng.normal(size=n)
+    w = rng.normal(size=n)
+dev = torch.device("cuda")
+
+
+def run(device):
+    qt = torch.tensor(q, device=device, requires_grad=True)
+    yt = torch.tensor(y, device=device, requires_grad=True)
+    phi = fmm_potential(qt, yt, torch.tensor(x, device=device), kernel, 8, 64, wavek)
+    loss = (phi * torch.tensor(w, device=device)).real.sum()
+    loss.backward()
+    return phi, qt.grad, yt.grad
+
+
+phi_d, gq_d, gy_d = run(dev)
+libs = hip_runtimes()
+print("HIP runtimes in the process:", libs, flush=True)
+assert len(libs) == 1, libs
+assert phi_d.is_cuda and gq_d.is_cuda and gy_d.is_cuda
+phi_c, gq_c, gy_c = run("cpu")
+# same engine, same tree: the device path returns the CPU path's values
+assert torch.equal(phi_d.cpu(), phi_c) and torch.equal(gq_d.cpu(), gq_c) and torch.equal(gy_d.cpu(), gy_c)
+# torch's own GPU arithmetic beside the engine: a direct sum of the potential and of d loss / d q on the
+# GPU in float64 (torch's autograd through the direct sum)
+qt = torch.tensor(q, device=dev, requires_grad=True)
+r = torch.cdist(torch.tensor(x, device=dev), torch.tensor(y, device=dev))
+if kernel == "helmholtz":
+    G = torch.exp(1j * wavek * r) / (4 * np.pi * r)
+else:
+    G = torch.exp(-wavek * r) / (4 * np.pi * r)
+phi_t = G @ qt
+(phi_t * torch.tensor(w, device=dev)).real.sum().backward()
+e_phi = (torch.linalg.norm(phi_d - phi_t.detach()) / torch.linalg.norm(phi_t.detach())).item()
+e_gq = (torch.linalg.norm(gq_d - qt.grad) / torch.linalg.norm(qt.grad)).item()
+print("device tensors: phi vs torch direct %.2e, grad q %.2e" % (e_phi, e_gq), flush=True)
+assert e_phi < 1e-6 and e_gq < 1e-6, (e_phi, e_gq)
+print("DEVICE_OK", flush=True)
+"""
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kernel", ["modified_helmholtz", "helmholtz"])
+def test_torch_autograd_device_tensors(require_gpu, kernel):
+    """GPU tensors through the autograd function, in a fresh process that imports torch first (one HIP
+    runtime): outputs and gradients come back on the GPU, equal to the CPU-tensor path bit for bit, and
+    match torch's own float64 direct sum (and its autograd) computed on the GPU beside the engine."""
+    pytest.importorskip("torch")
+    import subprocess
+    r = subprocess.run([sys.executable, "-c", _DEVICE_SCRIPT, os.path.join(ROOT, "exafmm-t_amd"), kernel,
+                        str(WAVEK[kernel])], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=240)
+    print(r.stdout[-3000:], flush=True)
+    assert r.returncode == 0 and "DEVICE_OK" in r.stdout
+
+
+def test_autograd_refuses_device_tensors_with_two_runtimes(monkeypatch):
+    """With two HIP runtimes in the process, GPU tensors are refused with an explanation (checked on the
+    CPU with a stand-in tensor and a patched runtime list)."""
+    torch = pytest.importorskip("torch")
+    from exafmm import autograd as ag
+
+    class FakeCuda:
+        is_cuda = True
+
+    monkeypatch.setattr(ag, "hip_runtimes", lambda: ["/a/libamdhip64.so", "/b/libamdhip64.so.7"])
+    with pytest.raises(RuntimeError, match="import torch before exafmm"):
+        ag._check_device(torch.zeros(1), FakeCuda())
+    monkeypatch.setattr(ag, "hip_runtimes", lambda: ["/a/libamdhip64.so"])
+    ag._check_device(torch.zeros(1), FakeCuda())
