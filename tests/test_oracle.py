@@ -126,12 +126,12 @@ def _full_tree(kernel):
     return x
 
 
-@pytest.mark.parametrize("kid,wk", KERNELS)
-def test_fft_m2l_equals_dense_m2l(kid, wk):
+@pytest.mark.parametrize("kid,wk,p", [k + (4,) for k in KERNELS] + [(O.LAPLACE, 0.0, 8)])
+def test_fft_m2l_equals_dense_m2l(kid, wk, p):
     """oracle M2L (operator build + fft_up_equiv + hadamard + ifft_dn_check)
     equals the dense kernel matrix from every non-adjacent source child's
-    upward-equivalent surface to the target child's downward-check surface."""
-    p = 4
+    upward-equivalent surface to the target child's downward-check surface
+    (p = 4 for the three kernels, and p = 8, n1 = 16, for Laplace)."""
     name = {O.LAPLACE: "laplace", O.YUKAWA: "modified_helmholtz", O.HELMHOLTZ: "helmholtz"}[kid]
     x = _full_tree(name)
     import exafmm
@@ -181,8 +181,11 @@ def test_fft_m2l_equals_dense_m2l(kid, wk):
 
 
 @pytest.mark.parametrize("kernel,p,tol", [("laplace", 4, 5e-4), ("laplace", 6, 2e-5), ("modified_helmholtz", 5, 1e-4),
-                                          ("helmholtz", 5, 1e-4)])
+                                          ("helmholtz", 5, 1e-4), ("laplace", 8, 1e-7), ("laplace", 10, 5e-8),
+                                          ("modified_helmholtz", 8, 2e-7), ("helmholtz", 8, 2e-7)])
 def test_oracle_fmm_against_direct(kernel, p, tol):
+    """The oracle's full evaluation against its direct sum (3,000 points, ncrit 32; measured at p = 8 / 10:
+    Laplace phi 2.8e-8 / 2.4e-8, grad 1.8e-7 / 1.5e-7; Yukawa and Helmholtz p = 8 phi 5.1e-8 / 3.6e-8)."""
     pr = make_problem(kernel, 3000, p, ncrit=32, dist="cube", seed=1)
     _fmm_vs_direct(pr, p, tol)
 
