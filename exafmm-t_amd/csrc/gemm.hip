@@ -290,19 +290,6 @@ void launch_zero(double* p, size_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_zero, dim3(unsigned(blocks)), dim3(256), 0, s, p, n);
 }
 
-__global__ void k_axpy(double* __restrict__ y, const double* __restrict__ x, size_t n) {
-  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  const size_t stride = size_t(gridDim.x) * blockDim.x;
-  for (; i < n; i += stride) y[i] += x[i];
-}
-
-void launch_axpy(double* y, const double* x, size_t n, hipStream_t s) {
-  if (n == 0) return;
-  size_t blocks = (n + 255) / 256;
-  if (blocks > 65536) blocks = 65536;
-  hipLaunchKernelGGL(k_axpy, dim3(unsigned(blocks)), dim3(256), 0, s, y, x, n);
-}
-
 __global__ __launch_bounds__(256) void k_charge_sums(const double* __restrict__ q, int n, bool cplx,
                                                      double* __restrict__ out) {
   __shared__ double red[3][256];

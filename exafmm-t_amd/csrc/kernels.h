@@ -160,7 +160,6 @@ void launch_ifft_dn(const M2LDev& m, const DevTree& t, const int* trg_node, cons
 size_t m2l_lds_bytes(const M2LDev& m);
 
 void launch_zero(double* p, size_t n, hipStream_t s);
-void launch_axpy(double* y, const double* x, size_t n, hipStream_t s);  // y += x
 // {Re sum q, Im sum q, sum |q|} of n charges (complex: interleaved pairs),
 // in a fixed order: per-block partial sums in out[0 .. 3 CHARGE_SUM_BLOCKS),
 // their total in out[3 CHARGE_SUM_BLOCKS .. + 3)
