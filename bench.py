@@ -215,10 +215,24 @@ def cpu_baseline(prob, p, ncrit, n):
                       "ncrit=%d, FP64), %.2f s; stages (s) %s" % (n, p, ncrit, dt, json.dumps(st))}
 
 
+def kernel_source_sha256(name, path=None):
+    """sha256 of the text of the kernel `name` (from its `void name(` line to the closing brace at column
+    0) in the engine's source: the key that ties a committed counter measurement to the code it measured."""
+    import hashlib
+    path = path or os.path.join(ROOT, "exafmm-t_amd", "csrc", "m2l.hip")
+    with open(path) as f:
+        src = f.read()
+    i = src.find("void %s(" % name)
+    if i < 0:
+        return None
+    j = src.find("\n}\n", i)
+    return hashlib.sha256(src[i:j + 3].encode()).hexdigest()
+
+
 def measured_traffic(dom, mixed, a):
     """HBM bytes per launch of the dominant kernel from the committed FETCH_SIZE / WRITE_SIZE
     measurement of the same workload (profiles/), scaled by the copy-kernel calibration; None when
-    no measurement matches this kernel and workload."""
+    no measurement matches this kernel, its current source text and the workload."""
     path = os.path.join(ROOT, "profiles", "laplace_N%s_p%d_hadamard_traffic.json" % (
         "2M" if a.n == 2_000_000 else str(a.n), a.p))
     if dom != "HAD" or a.ncrit != 128 or not os.path.exists(path):
@@ -227,6 +241,8 @@ def measured_traffic(dom, mixed, a):
         t = json.load(f)
     want = ("k_hadamard_h" if had16_active() else "k_hadamard<float>") if mixed else "k_hadamard<double>"
     if t.get("kernel") != want:
+        return None
+    if t.get("kernel_source_sha256") != kernel_source_sha256(want.split("<")[0]):
         return None
     c = t["calibration"]
     return 1024.0 * (t["fetch_kib_per_dispatch"] * c["fetch_scale"] + t["write_kib_per_dispatch"] * c["write_scale"])

@@ -57,3 +57,24 @@ def test_more_than_one_gpu_is_refused(argv, env):
                        env=dict(os.environ, **env), timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_committed_traffic_matches_the_kernel_source():
+    """The Hadamard traffic that bench.py reports comes from a committed FETCH_SIZE / WRITE_SIZE
+    measurement; it is keyed by the sha256 of the kernel's source text, so it is reported only while the
+    kernel is the one that was measured (a changed kernel reports null until it is measured again)."""
+    import argparse
+    a = argparse.Namespace(n=2_000_000, p=10, ncrit=128)
+    t = bench.measured_traffic("HAD", True, a)
+    assert t is not None and 2.0e9 < t < 6.0e9
+    h = bench.kernel_source_sha256("k_hadamard_h")
+    assert h is not None and len(h) == 64
+    src = os.path.join(os.path.dirname(bench.__file__), "exafmm-t_amd", "csrc", "m2l.hip")
+    text = open(src).read()
+    import tempfile
+    with tempfile.NamedTemporaryFile("w", suffix=".hip", delete=False) as f:
+        f.write(text.replace("void k_hadamard_h(", "void k_hadamard_h(\n", 1))
+    try:
+        assert bench.kernel_source_sha256("k_hadamard_h", f.name) != h
+    finally:
+        os.unlink(f.name)
